@@ -1,0 +1,183 @@
+"""Benchmark: split-step BPM throughput on one MI355X.
+
+Workload (one "step" = one full propagation): 1024 x 1024 grid x 1024 planes
+(1023 z-steps), voxel 0.1 um, lam 0.5 um, n0 1.33, complex64 paraxial
+Gaussian beam (w0 3 um) focused 20 um into the volume, float32 dn of 400
+seeded spheres (r ~ U(1,4) um, dn ~ U(.01,.05), overlaps summed); Bpm3d
+propagate path (static float64->complex64 propagator, phase screen per
+plane); outputs the exit field and the summed intensity sum_z |u_z|^2.
+Inputs are resident in HBM before the timed region; each step copies u0
+into the working plane and runs the whole z-loop.
+
+Prints one JSON line.  value = voxel-steps/s = Nx*Ny*(Nz-1) per step / time.
+roofline: the dominant kernel's algorithmic bytes per launch (row pass:
+plane read+write 16 B + dn 4 B per voxel; column pass: 16 B per voxel)
+divided by its average launch duration from HIP events recorded around
+every launch during the timed region.
+cpu_baseline: the numpy complex64 restatement of the same loop (the oracle,
+single-threaded numpy FFT) timed on a bounded sample of the workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nx", type=int, default=1024)
+    ap.add_argument("--nz", type=int, default=1024)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-sample-planes", type=int, default=65,
+                    help="planes of the workload the CPU baseline propagates")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-debye", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(u0, dn, units, lam, n0, planes):
+    """Time the oracle's complex64 restatement of the Bpm3d loop on the host."""
+    from oracle import bpm_oracle as O
+    Ny, Nx = u0.shape
+    sub = np.ascontiguousarray(dn[:planes])
+    t0 = time.perf_counter()
+    O.propagate_single((Nx, Ny, planes), units, lam, n0, sub, u0, return_full=False)
+    dt = time.perf_counter() - t0
+    vs = Nx * Ny * (planes - 1) / dt
+    return {"value": vs, "unit": "voxel-steps/s", "cores": 1, "kind": "port",
+            "sample": "%dx%d grid x %d planes (%d steps) of the same workload, numpy "
+                      "complex64 restatement (oracle.bpm_oracle.propagate_single), %.1f s"
+                      % (Nx, Ny, planes, planes - 1, dt)}
+
+
+def debye_extra(torch):
+    """The 256^3 Debye-Wolf workload (NA 1.1, n0 1.33, 512 theta steps)."""
+    from bpm_amd import psf as Q
+    from bpm_amd import _lib
+    shape, units, lam, NA, n0, S = (256, 256, 256), (.05,) * 3, .5, 1.1, 1.33, 512
+    Q.focus_field_debye_device(shape, units, lam, NA, n0, S)
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Q.focus_field_debye_device(shape, units, lam, NA, n0, S)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    samples = 129 ** 3 * (S + 1)
+    return {"workload": "Debye-Wolf 256^3, NA 1.1, n0 1.33, 512 theta steps, x-polarised",
+            "ms": dt * 1e3, "theta_samples_per_s": samples / dt,
+            "note": "octant quadrature + full-grid mirror, outputs on device"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(independent jobs would shard by job with no communication); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    import ctypes
+    import torch
+    from bpm_amd import _lib
+    from bpm_amd import bpm3d as B
+    from bpm_amd.workloads import plain_large_workload
+
+    Nx = Ny = a.nx
+    Nz = a.nz
+    units, lam, n0 = (.1, .1, .1), .5, 1.33
+    t0 = time.perf_counter()
+    u0, dn = plain_large_workload(seed=a.seed, size=(Nx, Ny, Nz), units=units, lam=lam, n0=n0)
+    gen_s = time.perf_counter() - t0
+    print("workload generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
+
+    dev = torch.device("cuda", 0)
+    dn_g = torch.from_numpy(dn).to(dev)
+    u0_g = torch.from_numpy(u0).to(dev).reshape(1, Ny, Nx)
+    plane = torch.empty_like(u0_g)
+    acc = torch.empty((1, Ny, Nx), dtype=torch.float32, device=dev)
+    H = B.propagator_table(Nx, Ny, units, lam, n0, _lib.H_F64)
+    plan = B.get_plan(Nx, Ny)
+    lib = _lib.load()
+
+    def step():
+        plane.copy_(u0_g)
+        B.propagate_fields(plane, Nz, units, lam, n0, dn=dn_g, dn_kind=_lib.DN_F32,
+                           mean_mode="ignore", H=H, out_sum=acc)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    _lib.check(lib.bpmh_plan_set_timing(plan, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms = (ctypes.c_float * 4)()
+    cnt = (ctypes.c_int * 4)()
+    _lib.check(lib.bpmh_plan_kernel_times(plan, ms, cnt))
+    _lib.check(lib.bpmh_plan_set_timing(plan, 0))
+
+    ms_step = el / a.steps * 1e3
+    vsteps = Nx * Ny * (Nz - 1)
+    value = vsteps / (ms_step * 1e-3)
+    row_avg = ms[0] / max(cnt[0], 1)
+    col_avg = ms[1] / max(cnt[1], 1)
+    row_bytes = (16. + 20. * (Nz - 1)) / Nz * Nx * Ny
+    col_bytes = 16. * Nx * Ny
+    kern = {"row_pass": {"avg_us": row_avg * 1e3, "launches": cnt[0], "total_ms": ms[0],
+                         "alg_bytes": row_bytes, "GBps": row_bytes / (row_avg * 1e-3) / 1e9},
+            "col_pass": {"avg_us": col_avg * 1e3, "launches": cnt[1], "total_ms": ms[1],
+                         "alg_bytes": col_bytes, "GBps": col_bytes / (col_avg * 1e-3) / 1e9}}
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    ach = kern[dom]["GBps"]
+    step_gbs = 36. * vsteps / (ms_step * 1e-3) / 1e9
+
+    out = {
+        "metric": "voxel-steps/s (split-step BPM, 1024^2 grid)",
+        "value": value,
+        "unit": "voxel-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": "plain BPM %dx%dx%d, voxel 0.1um, lam 0.5um, n0 1.33, "
+                               "Gaussian beam w0 3um focused 20um deep, 400 seeded spheres, "
+                               "Bpm3d propagate path; exit field + summed intensity"
+                               % (Nx, Ny, Nz),
+                   "nx": Nx, "ny": Ny, "nz": Nz, "batch": 1, "seed": a.seed},
+        "hbm_frac_step": step_gbs / HBM_PEAK_GBS,
+        "step_effective_GBps": step_gbs,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None},
+        "kernels": kern,
+    }
+    if not a.no_debye:
+        out["debye"] = debye_extra(torch)
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(u0, dn, units, lam, n0, min(a.cpu_sample_planes, Nz))
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

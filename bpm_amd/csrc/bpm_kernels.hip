@@ -1,0 +1,766 @@
+// Split-step beam propagation on gfx950: fused row/column FFT passes.
+//
+// One propagation step z-1 -> z of the reference loop (bpm/bpm/bpm_3d.py:283-382;
+// bpm/bpmclass/_bpm3d_ocl.py:100-126) is
+//     u_z = screen_z( IFFT2( H . FFT2(u_{z-1}) ) )
+// and is executed as two passes over the (Ny,Nx) complex64 plane that stays
+// in HBM between them, holding F_x u (the plane transformed along x only):
+//   column pass (one work-group per tile of COLS columns):
+//       FFT_y -> multiply by H (table or on-the-fly formula) -> IFFT_y
+//   row pass (one work-group per LINES rows):
+//       IFFT_x -> 1/(NxNy) -> phase screen dn[z] and taper -> outputs
+//       (field/intensity slice, running sum of |u|^2) -> FFT_x
+// so each step reads and writes the plane twice and reads dn[z] once.  The
+// default mean_mode "weighted" of bpm_3d needs the k-space reduction of
+// bpm_3d.py:302-303 before H, so its column pass is split in two (A: FFT_y
+// and per-tile partial sums; B: H(n0+dn0) and IFFT_y).
+#include "common.hpp"
+#include "fft.hpp"
+#include "bpm_hip.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace bpmfft;
+
+// ----------------------------------------------------------------------------
+// error reporting
+// ----------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+int bpmh_set_error(int code, const char* msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      char b_[256];                                                          \
+      snprintf(b_, sizeof b_, "%s: %s", #expr, hipGetErrorString(e_));       \
+      return bpmh_set_error(BPMH_ERR_HIP, b_);                               \
+    }                                                                        \
+  } while (0)
+
+extern "C" const char* bpmh_last_error(void) { return g_err.c_str(); }
+extern "C" int bpmh_version(void) { return BPMH_VERSION; }
+
+// ----------------------------------------------------------------------------
+// per-size launch configuration
+// ----------------------------------------------------------------------------
+template <int N>
+struct Cfg {
+  static constexpr int E = (N <= 32) ? 4 : 8;
+  static constexpr int T = N / E;
+  static constexpr int LINES = (256 / T) > 16 ? 16 : ((256 / T) < 1 ? 1 : 256 / T);
+  static constexpr int COLS0 = 512 / T;
+  static constexpr int COLS = N >= 4096 ? 1 : (COLS0 > 16 ? 16 : (COLS0 < 2 ? 2 : COLS0));
+  static constexpr int ROW_THREADS = LINES * T;
+  static constexpr int COL_THREADS = COLS * T;
+};
+
+// ----------------------------------------------------------------------------
+// device helpers
+// ----------------------------------------------------------------------------
+BPMH_DEV float2 load_dn(const void* dn, int kind, size_t idx) {
+  if (kind == BPMH_DN_F32) return make_float2(((const float*)dn)[idx], 0.f);
+  if (kind == BPMH_DN_F16) return make_float2(__half2float(((const __half*)dn)[idx]), 0.f);
+  return ((const float2*)dn)[idx];
+}
+
+// FFTFREQ(i,N,d) of bpm_3d_kernels.cl:6 in float32
+BPMH_DEV float fftfreq_f32(int i, int N, float d) {
+  const int half = (N - 1) / 2 + 1;
+  const float num = (i < half) ? (float)i : (float)((i - half) - N / 2);
+  return f_div(f_div(num, (float)N), d);
+}
+
+BPMH_DEV float kcomp2_f32(int i, int N, float d) {
+  const float k = f_mul(2.f * BPMH_PI_F, fftfreq_f32(i, N, d));
+  return f_mul(k, k);
+}
+
+// H for one (kx^2, ky^2) with a = ((n*n)*k0)*k0, as compute_propagator
+// (bpm_3d_kernels.cl:419-430): (tmp>=0) * exp(-i dz sqrt(tmp)).
+BPMH_DEV float2 prop_f32(float kx2, float ky2, float a, float dz) {
+  const float tmp = f_sub(f_sub(a, kx2), ky2);
+  const float h0 = tmp < 0.f ? 0.f : __fsqrt_rn(tmp);
+  const float2 cs = sincos_f32_via_f64(f_mul(-dz, h0));
+  return tmp >= 0.f ? cs : make_float2(0.f, 0.f);
+}
+
+BPMH_DEV float n_term_f32(float n, float k0) { return f_mul(f_mul(f_mul(n, n), k0), k0); }
+
+// dn0 = dn_sum/u_sum from per-tile partials (x = dn part, y = u part).
+// Lane p sums partials p, p+64, ...; the xor butterfly then gives every lane
+// (and every wave of every work-group) the bit-identical total.
+BPMH_DEV float weighted_dn0(const float2* __restrict__ wp, int nparts) {
+  const int lane = threadIdx.x & 63;
+  float sd = 0.f, su = 0.f;
+  for (int p = lane; p < nparts; p += 64) {
+    const float2 v = wp[p];
+    sd += v.x;
+    su += v.y;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    sd += __shfl_xor(sd, o, 64);
+    su += __shfl_xor(su, o, 64);
+  }
+  return sd / su;
+}
+
+// Deterministic block sum of a float2 (wave butterfly, then wave 0 adds the
+// per-wave values in order).  Uses red[] of at least 16 float2 in LDS.
+template <int NT>
+BPMH_DEV float2 block_sum2(float2 v, float2* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    v.x += __shfl_xor(v.x, o, 64);
+    v.y += __shfl_xor(v.y, o, 64);
+  }
+  constexpr int NW = (NT + 63) / 64;
+  if constexpr (NW == 1) return v;
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  float2 s = red[0];
+#pragma unroll
+  for (int i = 1; i < NW; ++i) s = c_add(s, red[i]);
+  __syncthreads();
+  return s;
+}
+
+// ----------------------------------------------------------------------------
+// row pass
+// ----------------------------------------------------------------------------
+struct RowArgs {
+  float2* plane;           // (rows = batch*ny, nx)
+  const float2* tw;        // exp(-2 pi i m/nx)
+  const void* dn;          // (nz, ny, nx) per batch item, kind dn_kind
+  long long dn_bstride;    // elements between batch items' dn (0: shared)
+  int dn_kind;
+  int ny, nz, z;
+  float unit_k;            // float32(k0*dz)
+  float inv_n;             // 1/(nx*ny)
+  int absorb;
+  int dn0_mode;            // BPMH_MEAN_*: 0 none, 1 per-plane array, 2 weighted
+  const float* dn0_arr;    // simple mode: float32 dn0 per plane
+  const float2* wpart;     // weighted: per-tile partials, nparts per item
+  int nparts;
+  float2* out_field;       // (batch, nz, ny, nx) or null
+  float* out_intens;       // (batch, nz, ny, nx) or null
+  float* acc;              // (batch, ny, nx) or null: sum_z |u_z|^2
+  int acc_init;
+  int inverse;             // input is F_x u: inverse FFT first
+  int screen;              // apply dn[z] screen
+  int forward;             // forward FFT_x before storing
+};
+
+template <int NX>
+__global__ void __launch_bounds__(Cfg<NX>::ROW_THREADS)
+k_row(RowArgs a) {
+  constexpr int E = Cfg<NX>::E, T = Cfg<NX>::T, LINES = Cfg<NX>::LINES;
+  __shared__ float2 lds[LINES][lds_len<NX>()];
+  const int line = threadIdx.x / T;
+  const int t = threadIdx.x - line * T;
+  const int row = blockIdx.x * LINES + line;  // in [0, batch*ny)
+  const int b = row / a.ny;
+  const int y = row - b * a.ny;
+  float2* p = a.plane + (size_t)row * NX;
+
+  float2 v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = p[t + m * T];
+
+  if (a.inverse) {
+    line_fft<NX, E, true>(v, lds[line], t, a.tw);
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = c_scale(v[m], a.inv_n);
+  }
+
+  if (a.screen) {
+    float dn0 = 0.f;
+    if (a.dn0_mode == BPMH_MEAN_SIMPLE) dn0 = a.dn0_arr[a.z];
+    else if (a.dn0_mode == BPMH_MEAN_WEIGHTED) dn0 = weighted_dn0(a.wpart + (size_t)b * a.nparts, a.nparts);
+    const size_t slice = ((size_t)a.z * a.ny + y) * NX + (size_t)b * a.dn_bstride;
+    const int disty = min(a.ny - y - 1, y);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int x = t + m * T;
+      const float2 d = load_dn(a.dn, a.dn_kind, slice + x);
+      if (a.dn_kind == BPMH_DN_C64) {
+        // cexp((0,-k)*(dn - dn0)) = e^{k Im} (cos(-k Re), sin(-k Re)), no taper
+        const float re = f_mul(a.unit_k, d.y);
+        const float im = f_mul(-a.unit_k, f_sub(d.x, dn0));
+        float s, c;
+        sincosf(im, &s, &c);
+        const float mag = expf(re);
+        v[m] = c_mul(v[m], make_float2(mag * c, mag * s));
+      } else {
+        const float ph = f_mul(-a.unit_k, f_sub(d.x, dn0));
+        float s, c;
+        sincosf(ph, &s, &c);
+        float2 r = c_mul(v[m], make_float2(c, s));
+        const int dist = min(min(NX - x - 1, x), disty);
+        if (dist < a.absorb) {
+          const float arg = f_div(f_mul(BPMH_PI_F, (float)dist), (float)a.absorb);
+          const float tp = f_mul(0.5f, f_sub(1.f, (float)cos((double)arg)));
+          r = make_float2(r.x * tp, r.y * tp);
+        }
+        v[m] = r;
+      }
+    }
+  }
+
+  const size_t plane_off = (size_t)y * NX;
+  if (a.out_field) {
+    float2* o = a.out_field + ((size_t)b * a.nz + a.z) * a.ny * NX + plane_off;
+#pragma unroll
+    for (int m = 0; m < E; ++m) o[t + m * T] = v[m];
+  }
+  if (a.out_intens) {
+    float* o = a.out_intens + ((size_t)b * a.nz + a.z) * a.ny * NX + plane_off;
+#pragma unroll
+    for (int m = 0; m < E; ++m) o[t + m * T] = v[m].x * v[m].x + v[m].y * v[m].y;
+  }
+  if (a.acc) {
+    float* o = a.acc + (size_t)b * a.ny * NX + plane_off;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const float i2 = v[m].x * v[m].x + v[m].y * v[m].y;
+      o[t + m * T] = a.acc_init ? i2 : o[t + m * T] + i2;
+    }
+  }
+
+  if (a.forward) line_fft<NX, E, false>(v, lds[line], t, a.tw);
+#pragma unroll
+  for (int m = 0; m < E; ++m) p[t + m * T] = v[m];
+}
+
+// ----------------------------------------------------------------------------
+// column pass
+// ----------------------------------------------------------------------------
+struct ColArgs {
+  float2* plane;           // (batch, ny, nx)
+  const float2* tw;        // exp(-2 pi i m/ny)
+  int nx, nz, z;
+  int hmode;               // 0: table H, 1: formula with n_eff[z], 2: formula with n0+weighted dn0
+  const float2* H;         // (ny, nx)
+  const float* n_eff;      // per plane, float32
+  float n0, k0, dx, dy, dz;
+  const void* dn;          // weighted: slice 0 of each item's dn is the weight
+  long long dn_bstride;
+  int dn_kind;
+  float2* wpart;           // weighted partials (batch, ntiles)
+  const float* sw;         // scattering weights (ny, nx) or null
+  const float* gw;
+  const float2* pw;        // plane-wave DC per plane (nz)
+  float2* spart;           // scattering partials (batch, nz, ntiles)
+};
+
+// PHASE 0: FFT_y, H, IFFT_y.  PHASE 1: FFT_y, weighted partials, store.
+// PHASE 2: weighted dn0, H(n0+dn0), IFFT_y.
+template <int NY, int PHASE>
+__global__ void __launch_bounds__(Cfg<NY>::COL_THREADS)
+k_col(ColArgs a) {
+  constexpr int E = Cfg<NY>::E, T = Cfg<NY>::T, COLS = Cfg<NY>::COLS;
+  constexpr int NT = Cfg<NY>::COL_THREADS;
+  __shared__ float2 lds[COLS][lds_len<NY>()];
+  __shared__ float2 red[16];
+  const int ntiles = a.nx / COLS;
+  const int bx = blockIdx.x;
+  // adjacent tiles share cache lines: keep them on one XCD (blocks are
+  // dealt round-robin over the 8 XCDs)
+  const int tile = (ntiles % 8 == 0) ? (bx & 7) * (ntiles >> 3) + (bx >> 3) : bx;
+  const int b = blockIdx.y;
+  const int c = threadIdx.x % COLS;
+  const int t = threadIdx.x / COLS;
+  const int x = tile * COLS + c;
+  float2* p = a.plane + (size_t)b * NY * a.nx + x;
+
+  float2 v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = p[(size_t)(t + m * T) * a.nx];
+
+  if constexpr (PHASE != 2) line_fft<NY, E, false>(v, lds[c], t, a.tw);
+
+  if constexpr (PHASE == 1) {
+    const size_t s0 = (size_t)b * a.dn_bstride;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int ky = t + m * T;
+      const float au = hypotf(v[m].x, v[m].y);
+      const float d = a.dn ? load_dn(a.dn, a.dn_kind, s0 + (size_t)ky * a.nx + x).x : 0.f;
+      acc.x += au * d;
+      acc.y += au;
+    }
+    acc = block_sum2<NT>(acc, red);
+    if (threadIdx.x == 0) a.wpart[(size_t)b * ntiles + tile] = acc;
+  } else {
+    float an = 0.f, kx2 = 0.f;
+    if (a.hmode != 0) {
+      float n = (a.hmode == 1) ? a.n_eff[a.z]
+                               : f_add(a.n0, weighted_dn0(a.wpart + (size_t)b * ntiles, ntiles));
+      an = n_term_f32(n, a.k0);
+      kx2 = kcomp2_f32(x, a.nx, a.dx);
+    }
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int ky = t + m * T;
+      const float2 h = (a.hmode == 0) ? a.H[(size_t)ky * a.nx + x]
+                                      : prop_f32(kx2, kcomp2_f32(ky, NY, a.dy), an, a.dz);
+      v[m] = c_mul(v[m], h);
+    }
+    if (a.sw) {
+      // scattering reductions of bpm_3d.py:243-247,335-341 on the k-space
+      // plane after H: sum w |U - delta_{k,0} plain|^2
+      float2 acc = make_float2(0.f, 0.f);
+      const float2 pl = a.pw[a.z];
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int ky = t + m * T;
+        float2 f = v[m];
+        if (ky == 0 && x == 0) f = c_sub(f, pl);
+        const float af = hypotf(f.x, f.y);
+        const float a2 = af * af;
+        const size_t k = (size_t)ky * a.nx + x;
+        acc.x += a.sw[k] * a2;
+        acc.y += a.gw[k] * a2;
+      }
+      acc = block_sum2<NT>(acc, red);
+      if (threadIdx.x == 0) a.spart[((size_t)b * a.nz + a.z) * ntiles + tile] = acc;
+    }
+    line_fft<NY, E, true>(v, lds[c], t, a.tw);
+  }
+
+#pragma unroll
+  for (int m = 0; m < E; ++m) p[(size_t)(t + m * T) * a.nx] = v[m];
+}
+
+// ----------------------------------------------------------------------------
+// propagator tables and small helpers
+// ----------------------------------------------------------------------------
+// kind F32: compute_propagator (bpm_3d_kernels.cl:408-432), float32 formula.
+// kind F64: host float64 propagator (bpm_3d.py:163-181, bpmclass/bpm3d.py:88-107)
+//           rounded to complex64.
+__global__ void k_prop_table(float2* H, int nx, int ny, int kind, double dx, double dy,
+                             double dz, double n0, double lam) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= nx) return;
+  const double k0 = 2. * M_PI / lam;
+  float2 h;
+  if (kind == BPMH_H_F32) {
+    const float an = n_term_f32((float)n0, (float)k0);
+    h = prop_f32(kcomp2_f32(i, nx, (float)dx), kcomp2_f32(j, ny, (float)dy), an, (float)dz);
+  } else {
+    // numpy: 2*pi*fftfreq(n,d) = 2*pi*(m * (1/(n*d)))
+    const int mi = (i < (nx - 1) / 2 + 1) ? i : i - nx;
+    const int mj = (j < (ny - 1) / 2 + 1) ? j : j - ny;
+    const double kx = (2. * M_PI) * ((double)mi * (1.0 / (nx * dx)));
+    const double ky = (2. * M_PI) * ((double)mj * (1.0 / (ny * dy)));
+    const double tmp = (n0 * n0) * (k0 * k0) - kx * kx - ky * ky;
+    if (tmp < 0.) {
+      h = make_float2(0.f, 0.f);
+    } else {
+      double s, c;
+      sincos(-dz * sqrt(tmp), &s, &c);
+      h = make_float2((float)c, (float)s);
+    }
+  }
+  H[(size_t)j * nx + i] = h;
+}
+
+__global__ void k_scatter_final(const float2* spart, float* out, int nz, int ntiles) {
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (z >= nz) return;
+  float2 s = make_float2(0.f, 0.f);
+  if (z > 0) {
+    const float2* q = spart + ((size_t)b * nz + z) * ntiles;
+    for (int i = 0; i < ntiles; ++i) s = c_add(s, q[i]);
+  }
+  out[((size_t)b * nz + z) * 2 + 0] = s.x;
+  out[((size_t)b * nz + z) * 2 + 1] = s.y;
+}
+
+// ----------------------------------------------------------------------------
+// plans
+// ----------------------------------------------------------------------------
+// Kernel kinds timed by the optional HIP-event instrumentation.
+enum { TK_ROW = 0, TK_COL = 1, TK_COL_A = 2, TK_COL_B = 3, TK_N = 4 };
+
+struct bpmh_plan_s {
+  int nx, ny;
+  float2* tw_x;
+  float2* tw_y;
+  void* scratch;
+  size_t scratch_bytes;
+  int device;
+  // timing: events recorded around every launch of bpmh_propagate
+  int timing;
+  std::vector<hipEvent_t> ev;   // pairs (start, stop)
+  std::vector<int> ev_kind;     // kind of each pair
+  size_t ev_used;               // pairs recorded since the last read-out
+};
+
+static bool pow2_in_range(int n) { return n >= 16 && n <= 4096 && (n & (n - 1)) == 0; }
+
+// Twiddles exp(-2 pi i m/N).  Angles are reduced to the first octant so the
+// symmetric values (m = 0, N/8, N/4, ...) are exact; among the float32
+// neighbours of (cos, sin) the pair with |w|^2 closest to 1 is kept, since
+// a magnitude error of a twiddle is applied again on every step.
+static void make_twiddles(int N, std::vector<float2>& tw) {
+  tw.resize(N);
+  for (int m = 0; m < N; ++m) {
+    // exact octant reduction of the angle 2 pi m / N
+    const long long M = 8LL * m;  // angle in units of 2*pi/(8N): octant = M / N
+    const int oct = (int)(M / N);
+    const double frac = (double)(M - (long long)oct * N) / N;  // in [0,1)
+    double c, s;
+    const double phi = frac * (M_PI / 4.0);
+    const double cp = cos(phi), sp = sin(phi), cq = cos(M_PI / 4.0 - phi), sq = sin(M_PI / 4.0 - phi);
+    switch (oct & 7) {  // cos/sin of oct*pi/4 + phi
+      case 0: c = cp; s = sp; break;
+      case 1: c = sq; s = cq; break;
+      case 2: c = -sp; s = cp; break;
+      case 3: c = -cq; s = sq; break;
+      case 4: c = -cp; s = -sp; break;
+      case 5: c = -sq; s = -cq; break;
+      case 6: c = sp; s = -cp; break;
+      default: c = cq; s = -sq; break;
+    }
+    if (frac == 0.0) {  // exact multiples of pi/4
+      const double r = sqrt(0.5);
+      const double cc[8] = {1, r, 0, -r, -1, -r, 0, r};
+      const double ss[8] = {0, r, 1, r, 0, -r, -1, -r};
+      c = cc[oct & 7];
+      s = ss[oct & 7];
+    }
+    const float c0 = (float)c, s0 = (float)-s;
+    float bc = c0, bs = s0;
+    double best = 1e300;
+    for (int dc = -1; dc <= 1; ++dc) {
+      for (int ds = -1; ds <= 1; ++ds) {
+        const float cf = dc == 0 ? c0 : nextafterf(c0, dc > 0 ? 2.f : -2.f);
+        const float sf = ds == 0 ? s0 : nextafterf(s0, ds > 0 ? 2.f : -2.f);
+        const double mag = fabs((double)cf * cf + (double)sf * sf - 1.0);
+        const double ang = fabs(atan2((double)sf, (double)cf) - atan2(-s, c));
+        const double score = mag + 0.25 * ang;
+        if (score < best) { best = score; bc = cf; bs = sf; }
+      }
+    }
+    tw[m] = make_float2(bc, bs);
+  }
+}
+
+extern "C" int bpmh_plan_create(int nx, int ny, bpmh_plan* out) {
+  if (!out) return bpmh_set_error(BPMH_ERR_INVALID, "null plan pointer");
+  *out = nullptr;
+  if (!pow2_in_range(nx) || !pow2_in_range(ny))
+    return bpmh_set_error(BPMH_ERR_UNSUPPORTED,
+                          "grid sizes must be powers of two in [16, 4096]");
+  bpmh_plan p = new bpmh_plan_s();
+  p->nx = nx;
+  p->ny = ny;
+  p->scratch = nullptr;
+  p->scratch_bytes = 0;
+  p->timing = 0;
+  p->ev_used = 0;
+  (void)hipGetDevice(&p->device);
+  std::vector<float2> tx, ty;
+  make_twiddles(nx, tx);
+  make_twiddles(ny, ty);
+  if (hipMalloc(&p->tw_x, nx * sizeof(float2)) != hipSuccess ||
+      hipMalloc(&p->tw_y, ny * sizeof(float2)) != hipSuccess) {
+    delete p;
+    return bpmh_set_error(BPMH_ERR_NOMEM, "hipMalloc failed for twiddles");
+  }
+  if (hipMemcpy(p->tw_x, tx.data(), nx * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p->tw_y, ty.data(), ny * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(p->tw_x);
+    (void)hipFree(p->tw_y);
+    delete p;
+    return bpmh_set_error(BPMH_ERR_HIP, "hipMemcpy failed for twiddles");
+  }
+  *out = p;
+  return BPMH_OK;
+}
+
+extern "C" int bpmh_plan_destroy(bpmh_plan p) {
+  if (!p) return BPMH_OK;
+  (void)hipFree(p->tw_x);
+  (void)hipFree(p->tw_y);
+  if (p->scratch) (void)hipFree(p->scratch);
+  for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
+  delete p;
+  return BPMH_OK;
+}
+
+extern "C" int bpmh_plan_set_timing(bpmh_plan p, int enable) {
+  if (!p) return bpmh_set_error(BPMH_ERR_INVALID, "null plan");
+  p->timing = enable != 0;
+  p->ev_used = 0;
+  return BPMH_OK;
+}
+
+extern "C" int bpmh_plan_kernel_times(bpmh_plan p, float* ms, int* counts) {
+  if (!p || !ms || !counts) return bpmh_set_error(BPMH_ERR_INVALID, "null argument");
+  for (int k = 0; k < TK_N; ++k) {
+    ms[k] = 0.f;
+    counts[k] = 0;
+  }
+  for (size_t i = 0; i < p->ev_used; ++i) {
+    float t = 0.f;
+    HIP_TRY(hipEventSynchronize(p->ev[2 * i + 1]));
+    HIP_TRY(hipEventElapsedTime(&t, p->ev[2 * i], p->ev[2 * i + 1]));
+    ms[p->ev_kind[i]] += t;
+    counts[p->ev_kind[i]] += 1;
+  }
+  p->ev_used = 0;
+  return BPMH_OK;
+}
+
+// Reserve event pairs for n more timed launches (outside any stream capture).
+static int timing_reserve(bpmh_plan p, size_t n) {
+  const size_t need = 2 * (p->ev_used + n);
+  while (p->ev.size() < need) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    p->ev.push_back(e);
+  }
+  if (p->ev_kind.size() < need / 2) p->ev_kind.resize(need / 2);
+  return BPMH_OK;
+}
+
+static int plan_scratch(bpmh_plan p, size_t bytes, void** ptr) {
+  if (bytes > p->scratch_bytes) {
+    if (p->scratch) {
+      HIP_TRY(hipDeviceSynchronize());
+      HIP_TRY(hipFree(p->scratch));
+      p->scratch = nullptr;
+      p->scratch_bytes = 0;
+    }
+    if (hipMalloc(&p->scratch, bytes) != hipSuccess)
+      return bpmh_set_error(BPMH_ERR_NOMEM, "hipMalloc failed for plan scratch");
+    p->scratch_bytes = bytes;
+  }
+  *ptr = p->scratch;
+  return BPMH_OK;
+}
+
+extern "C" int bpmh_propagator_table(int nx, int ny, int kind, double dx, double dy, double dz,
+                                     double n0, double lam, float* H_, void* stream) {
+  float2* H = (float2*)H_;
+  if (!H) return bpmh_set_error(BPMH_ERR_INVALID, "null H");
+  if (nx <= 0 || ny <= 0) return bpmh_set_error(BPMH_ERR_INVALID, "bad grid");
+  if (kind != BPMH_H_F32 && kind != BPMH_H_F64)
+    return bpmh_set_error(BPMH_ERR_INVALID, "unknown propagator kind");
+  dim3 grid((nx + 255) / 256, ny);
+  hipLaunchKernelGGL(k_prop_table, grid, dim3(256), 0, (hipStream_t)stream, H, nx, ny, kind, dx,
+                     dy, dz, n0, lam);
+  HIP_TRY(hipGetLastError());
+  return BPMH_OK;
+}
+
+// ----------------------------------------------------------------------------
+// launchers
+// ----------------------------------------------------------------------------
+typedef void (*row_fn)(const RowArgs&, int batch, hipStream_t);
+typedef void (*col_fn)(const ColArgs&, int batch, hipStream_t);
+
+template <int N>
+static void launch_row(const RowArgs& a, int batch, hipStream_t s) {
+  const int rows = batch * a.ny;
+  hipLaunchKernelGGL(k_row<N>, dim3(rows / Cfg<N>::LINES), dim3(Cfg<N>::ROW_THREADS), 0, s, a);
+}
+template <int N, int PHASE>
+static void launch_col(const ColArgs& a, int batch, hipStream_t s) {
+  hipLaunchKernelGGL((k_col<N, PHASE>), dim3(a.nx / Cfg<N>::COLS, batch),
+                     dim3(Cfg<N>::COL_THREADS), 0, s, a);
+}
+
+static row_fn pick_row(int n) {
+  switch (n) {
+    case 16: return launch_row<16>;
+    case 32: return launch_row<32>;
+    case 64: return launch_row<64>;
+    case 128: return launch_row<128>;
+    case 256: return launch_row<256>;
+    case 512: return launch_row<512>;
+    case 1024: return launch_row<1024>;
+    case 2048: return launch_row<2048>;
+    case 4096: return launch_row<4096>;
+  }
+  return nullptr;
+}
+
+template <int PHASE>
+static col_fn pick_col(int n) {
+  switch (n) {
+    case 16: return launch_col<16, PHASE>;
+    case 32: return launch_col<32, PHASE>;
+    case 64: return launch_col<64, PHASE>;
+    case 128: return launch_col<128, PHASE>;
+    case 256: return launch_col<256, PHASE>;
+    case 512: return launch_col<512, PHASE>;
+    case 1024: return launch_col<1024, PHASE>;
+    case 2048: return launch_col<2048, PHASE>;
+    case 4096: return launch_col<4096, PHASE>;
+  }
+  return nullptr;
+}
+
+static int col_tiles(int nx, int ny) {
+  switch (ny) {
+    case 16: return nx / Cfg<16>::COLS;
+    case 32: return nx / Cfg<32>::COLS;
+    case 64: return nx / Cfg<64>::COLS;
+    case 128: return nx / Cfg<128>::COLS;
+    case 256: return nx / Cfg<256>::COLS;
+    case 512: return nx / Cfg<512>::COLS;
+    case 1024: return nx / Cfg<1024>::COLS;
+    case 2048: return nx / Cfg<2048>::COLS;
+    case 4096: return nx / Cfg<4096>::COLS;
+  }
+  return 0;
+}
+
+extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, const void* dn,
+                              int dn_kind, long long dn_batch_stride, int mean_mode,
+                              const float* H_, const float* n_eff, const float* dn0, float n0,
+                              float k0, float dx, float dy, float dz, float unit_k, int absorb,
+                              float* out_field_, float* out_intens, float* out_sum,
+                              const float* scat_w, const float* g_w, const float* plain_,
+                              float* out_scat, void* stream) {
+  float2* plane = (float2*)plane_;
+  const float2* H = (const float2*)H_;
+  float2* out_field = (float2*)out_field_;
+  const float2* plain = (const float2*)plain_;
+  if (!plan) return bpmh_set_error(BPMH_ERR_INVALID, "null plan");
+  if (!plane) return bpmh_set_error(BPMH_ERR_INVALID, "null plane");
+  if (batch < 1 || nz < 1) return bpmh_set_error(BPMH_ERR_INVALID, "batch and nz must be >= 1");
+  if (dn_kind < BPMH_DN_NONE || dn_kind > BPMH_DN_C64)
+    return bpmh_set_error(BPMH_ERR_INVALID, "unknown dn kind");
+  if (dn_kind != BPMH_DN_NONE && !dn) return bpmh_set_error(BPMH_ERR_INVALID, "null dn");
+  if (mean_mode < BPMH_MEAN_IGNORE || mean_mode > BPMH_MEAN_WEIGHTED)
+    return bpmh_set_error(BPMH_ERR_INVALID, "unknown mean mode");
+  if (mean_mode == BPMH_MEAN_IGNORE && !H)
+    return bpmh_set_error(BPMH_ERR_INVALID, "mean_mode ignore needs a propagator table");
+  if (mean_mode == BPMH_MEAN_SIMPLE && (!n_eff || !dn0))
+    return bpmh_set_error(BPMH_ERR_INVALID, "mean_mode simple needs n_eff and dn0 arrays");
+  if (absorb < 0) return bpmh_set_error(BPMH_ERR_INVALID, "absorbing width must be >= 0");
+  if (out_scat && (!scat_w || !g_w || !plain))
+    return bpmh_set_error(BPMH_ERR_INVALID, "scattering output needs weights and plane-wave array");
+  const int nx = plan->nx, ny = plan->ny;
+  if ((long long)batch * ny > (1LL << 30)) return bpmh_set_error(BPMH_ERR_INVALID, "batch too large");
+  hipStream_t s = (hipStream_t)stream;
+  const row_fn row = pick_row(nx);
+  const col_fn col0 = pick_col<0>(ny);
+  const col_fn colA = pick_col<1>(ny);
+  const col_fn colB = pick_col<2>(ny);
+  const int ntiles = col_tiles(nx, ny);
+
+  // scratch: weighted partials (batch*ntiles) + scattering partials (batch*nz*ntiles)
+  size_t n_w = (size_t)batch * ntiles;
+  size_t n_s = out_scat ? (size_t)batch * nz * ntiles : 0;
+  void* scr = nullptr;
+  int rc = plan_scratch(plan, (n_w + n_s + 1) * sizeof(float2), &scr);
+  if (rc) return rc;
+  float2* wpart = (float2*)scr;
+  float2* spart = wpart + n_w;
+
+  RowArgs ra;
+  memset(&ra, 0, sizeof ra);
+  ra.plane = plane;
+  ra.tw = plan->tw_x;
+  ra.dn = dn;
+  ra.dn_bstride = dn_batch_stride;
+  ra.dn_kind = dn_kind;
+  ra.ny = ny;
+  ra.nz = nz;
+  ra.unit_k = unit_k;
+  ra.inv_n = 1.0f / ((float)nx * (float)ny);
+  ra.absorb = absorb;
+  ra.dn0_mode = (dn_kind == BPMH_DN_NONE) ? 0 : mean_mode;
+  ra.dn0_arr = dn0;
+  ra.wpart = wpart;
+  ra.nparts = ntiles;
+  ra.out_field = out_field;
+  ra.out_intens = out_intens;
+  ra.acc = out_sum;
+
+  ColArgs ca;
+  memset(&ca, 0, sizeof ca);
+  ca.plane = plane;
+  ca.tw = plan->tw_y;
+  ca.nx = nx;
+  ca.nz = nz;
+  ca.hmode = mean_mode;  // 0 table, 1 n_eff array, 2 weighted
+  ca.H = H;
+  ca.n_eff = n_eff;
+  ca.n0 = n0;
+  ca.k0 = k0;
+  ca.dx = dx;
+  ca.dy = dy;
+  ca.dz = dz;
+  ca.dn = dn;
+  ca.dn_bstride = dn_batch_stride;
+  ca.dn_kind = dn_kind;
+  ca.wpart = wpart;
+  ca.sw = out_scat ? scat_w : nullptr;
+  ca.gw = g_w;
+  ca.pw = plain;
+  ca.spart = spart;
+
+  if (plan->timing) {
+    rc = timing_reserve(plan, (size_t)nz * 3 + 1);
+    if (rc) return rc;
+  }
+  auto timed = [&](int kind, auto&& launch) {
+    if (!plan->timing) {
+      launch();
+      return;
+    }
+    const size_t i = plan->ev_used++;
+    plan->ev_kind[i] = kind;
+    (void)hipEventRecord(plan->ev[2 * i], s);
+    launch();
+    (void)hipEventRecord(plan->ev[2 * i + 1], s);
+  };
+
+  // plane 0: outputs of u0, then FFT_x
+  ra.z = 0;
+  ra.acc_init = 1;
+  ra.inverse = 0;
+  ra.screen = 0;
+  ra.forward = nz > 1;
+  timed(TK_ROW, [&] { row(ra, batch, s); });
+  ra.acc_init = 0;
+  ra.inverse = 1;
+  ra.screen = dn_kind != BPMH_DN_NONE;
+  for (int z = 1; z < nz; ++z) {
+    ca.z = z;
+    if (mean_mode == BPMH_MEAN_WEIGHTED) {
+      timed(TK_COL_A, [&] { colA(ca, batch, s); });
+      timed(TK_COL_B, [&] { colB(ca, batch, s); });
+    } else {
+      timed(TK_COL, [&] { col0(ca, batch, s); });
+    }
+    ra.z = z;
+    ra.forward = z < nz - 1;
+    timed(TK_ROW, [&] { row(ra, batch, s); });
+  }
+  if (out_scat) {
+    hipLaunchKernelGGL(k_scatter_final, dim3((nz + 63) / 64, batch), dim3(64), 0, s, spart,
+                       out_scat, nz, ntiles);
+  }
+  HIP_TRY(hipGetLastError());
+  return BPMH_OK;
+}

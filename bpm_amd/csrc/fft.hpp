@@ -1,0 +1,154 @@
+// Register/LDS Stockham FFT for one line of N = 2^n complex float32 points.
+//
+// Algorithm: Stockham autosort, decimation in time (Govindaraju, Lloyd,
+// Dotsenko, Smith, Manferdelli, "High Performance Discrete Fourier Transforms
+// on Graphics Processors", SC'08).  Stage s has radix R and stride Ns; the
+// butterfly j reads x[j + r*N/R], twiddles by W_{Ns*R}^{r*(j mod Ns)}, and
+// writes y[(j - j mod Ns)*R + j mod Ns + r*Ns].
+//
+// Data layout: a line is processed by T = N/E threads; thread t holds the E
+// elements t + m*T (m = 0..E-1) in registers on entry and on exit, so the
+// first stage's loads and the last stage's stores are unit-stride across a
+// wavefront (coalesced when the line is contiguous, and C-wide segments for a
+// tile of C columns).  Stages: one radix-R0 stage (R0 = N / E^S, no
+// twiddles since Ns = 1) followed by S radix-E stages.  Between stages the
+// data crosses threads through a padded LDS line; the last stage is in place
+// in registers.
+#pragma once
+#include "common.hpp"
+
+namespace bpmfft {
+
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+
+// LDS slot for logical index p: one pad slot per 16 to break the bank
+// aliasing of the scattered Stockham writes at small Ns.
+BPMH_DEV int lpad(int p) { return p + (p >> 4); }
+template <int N>
+constexpr int lds_len() { return N + N / 16; }
+
+// s * 2^-1/2 with a two-term constant: the float32 rounding of 2^-1/2 alone
+// biases every W8^odd product by -1.7e-8, which compounds over thousands of
+// transforms; hi+lo keeps the product unbiased.
+BPMH_DEV float mul_rsqrt2(float s) {
+  const float hi = 0.70710677f;       // nearest float32 to 2^-1/2
+  const float lo = 1.21016175e-08f;   // 2^-1/2 - hi
+  return fmaf(s, lo, s * hi);
+}
+
+template <bool INV>
+BPMH_DEV void dft(float2 (&a)[2]) {
+  const float2 t = a[0];
+  a[0] = c_add(t, a[1]);
+  a[1] = c_sub(t, a[1]);
+}
+
+template <bool INV>
+BPMH_DEV void dft4v(float2& a0, float2& a1, float2& a2, float2& a3) {
+  const float2 s02 = c_add(a0, a2), d02 = c_sub(a0, a2);
+  const float2 s13 = c_add(a1, a3), d13 = c_rot90<INV>(c_sub(a1, a3));
+  a0 = c_add(s02, s13);
+  a2 = c_sub(s02, s13);
+  a1 = c_add(d02, d13);
+  a3 = c_sub(d02, d13);
+}
+
+template <bool INV>
+BPMH_DEV void dft(float2 (&a)[4]) { dft4v<INV>(a[0], a[1], a[2], a[3]); }
+
+template <bool INV>
+BPMH_DEV void dft(float2 (&a)[8]) {
+  float2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+  float2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+  dft4v<INV>(e0, e1, e2, e3);
+  dft4v<INV>(o0, o1, o2, o3);
+  if (!INV) {  // o_k *= exp(-2 pi i k/8)
+    o1 = make_float2(mul_rsqrt2(o1.x + o1.y), mul_rsqrt2(o1.y - o1.x));
+    o2 = c_rot90<false>(o2);
+    o3 = make_float2(mul_rsqrt2(o3.y - o3.x), -mul_rsqrt2(o3.x + o3.y));
+  } else {     // o_k *= exp(+2 pi i k/8)
+    o1 = make_float2(mul_rsqrt2(o1.x - o1.y), mul_rsqrt2(o1.x + o1.y));
+    o2 = c_rot90<true>(o2);
+    o3 = make_float2(-mul_rsqrt2(o3.x + o3.y), mul_rsqrt2(o3.x - o3.y));
+  }
+  a[0] = c_add(e0, o0); a[4] = c_sub(e0, o0);
+  a[1] = c_add(e1, o1); a[5] = c_sub(e1, o1);
+  a[2] = c_add(e2, o2); a[6] = c_sub(e2, o2);
+  a[3] = c_add(e3, o3); a[7] = c_sub(e3, o3);
+}
+
+// One Stockham stage.  tw[m] = exp(-2 pi i m / N) for m < N.
+template <int N, int E, int R, int NS, bool INV, bool LAST>
+BPMH_DEV void stage(float2 (&v)[E], float2* __restrict__ lds, int t,
+                    const float2* __restrict__ tw) {
+  constexpr int T = N / E;
+  constexpr int B = E / R;
+#pragma unroll
+  for (int q = 0; q < B; ++q) {
+    const int j = t + q * T;
+    const int k = j & (NS - 1);
+    float2 a[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[r] = v[q + r * B];
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const float2 w = tw[(r * k) * (N / (NS * R))];
+        a[r] = INV ? c_mulc(a[r], w) : c_mul(a[r], w);
+      }
+    }
+    dft<INV>(a);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q + r * B] = a[r];
+    } else {
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[lpad(base + r * NS)] = a[r];
+    }
+  }
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = lds[lpad(t + m * T)];
+    __syncthreads();
+  }
+}
+
+template <int N, int E, int NS, bool INV>
+struct RadixEStages {
+  BPMH_DEV static void run(float2 (&v)[E], float2* lds, int t, const float2* tw) {
+    constexpr bool LAST = (NS * E == N);
+    stage<N, E, E, NS, INV, LAST>(v, lds, t, tw);
+    if constexpr (!LAST) RadixEStages<N, E, NS * E, INV>::run(v, lds, t, tw);
+  }
+};
+
+template <int N, int E>
+struct Shape {
+  static constexpr int n = ilog2c(N);
+  static constexpr int e = ilog2c(E);
+  static constexpr int S = n / e;
+  static constexpr int R0 = 1 << (n - S * e);
+  static constexpr int T = N / E;
+  static_assert((1 << n) == N, "N must be a power of two");
+  static_assert((1 << e) == E && E >= 2 && E <= 8, "E must be 2, 4 or 8");
+  static_assert(N >= E * E, "N must hold at least two radix-E stages");
+};
+
+// Unnormalised DFT (INV=false: exp(-2 pi i jk/N); INV=true: exp(+2 pi i jk/N))
+// of the line held as v[m] = x[t + m*T]; result in the same layout.
+// Every thread of the work-group must call it (it synchronises the group).
+template <int N, int E, bool INV>
+BPMH_DEV void line_fft(float2 (&v)[E], float2* __restrict__ lds, int t,
+                       const float2* __restrict__ tw) {
+  using Sh = Shape<N, E>;
+  if constexpr (Sh::R0 > 1) {
+    stage<N, E, Sh::R0, 1, INV, false>(v, lds, t, tw);
+    RadixEStages<N, E, Sh::R0, INV>::run(v, lds, t, tw);
+  } else {
+    RadixEStages<N, E, 1, INV>::run(v, lds, t, tw);
+  }
+}
+
+}  // namespace bpmfft

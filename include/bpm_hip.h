@@ -1,0 +1,150 @@
+/* C ABI of libbpmhip.so: split-step BPM and vectorial Debye-Wolf on gfx950.
+ *
+ * Every entry point returns BPMH_OK (0) or a negative BPMH_ERR_* code and
+ * never throws; bpmh_last_error() holds a thread-local message for the last
+ * failure.  All array arguments are DEVICE pointers owned by the caller
+ * (C-order, x fastest; a complex64 array is a float* of interleaved
+ * (re, im) pairs); work is enqueued on
+ * the caller's hipStream_t (passed as void*, NULL = default stream) and is
+ * asynchronous.  No entry point takes a host array except where stated.
+ *
+ * The reference binds its device work through gputools/pyopencl
+ * (OCLProgram.run_kernel, fft, OCLReductionKernel); each declaration below
+ * names the reference interface it replaces.
+ */
+#ifndef BPM_HIP_H
+#define BPM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BPMH_VERSION 100
+
+#define BPMH_OK 0
+#define BPMH_ERR_INVALID -1      /* bad argument  (Python: ValueError) */
+#define BPMH_ERR_UNSUPPORTED -2  /* unsupported   (Python: NotImplementedError) */
+#define BPMH_ERR_HIP -3          /* HIP runtime error */
+#define BPMH_ERR_NOMEM -4        /* device allocation failed */
+
+/* dn kinds (bpm_3d.py:198-214) */
+#define BPMH_DN_NONE 0
+#define BPMH_DN_F32 1
+#define BPMH_DN_F16 2
+#define BPMH_DN_C64 3
+
+/* mean modes (bpm_3d.py:54,120,286-317) */
+#define BPMH_MEAN_IGNORE 0   /* static propagator table H */
+#define BPMH_MEAN_SIMPLE 1   /* H(n_eff[z]) and screen dn - dn0[z], per-plane arrays */
+#define BPMH_MEAN_WEIGHTED 2 /* dn0 = sum|U| dn_slice0 / sum|U| on device each step */
+
+/* propagator table kinds */
+#define BPMH_H_F32 0  /* compute_propagator, bpm/bpm/kernels/bpm_3d_kernels.cl:408-432 */
+#define BPMH_H_F64 1  /* host float64 H, bpm/bpmclass/bpm3d.py:88-107 (rounded to complex64) */
+
+typedef struct bpmh_plan_s* bpmh_plan;
+
+/* Library version (BPMH_VERSION). */
+int bpmh_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* bpmh_last_error(void);
+
+/* Plan for an (ny, nx) grid: FFT twiddles and step scratch.
+ * Replaces gputools fft_plan((Ny,Nx)) (bpm/bpm/bpm_3d.py:188,
+ * bpm/bpmclass/_bpm3d_ocl.py:43).  nx, ny: powers of two in [16, 4096]. */
+int bpmh_plan_create(int nx, int ny, bpmh_plan* out);
+int bpmh_plan_destroy(bpmh_plan plan);
+
+/* Measurement hooks (the reference only prints wall-clock StopWatch times,
+ * bpm/utils/utils.py:7-37, bpm/bpm/bpm_3d.py:140-142,384-386).  With timing
+ * enabled, bpmh_propagate records a HIP event pair around every kernel it
+ * launches; bpmh_plan_kernel_times waits for them and returns, per kernel
+ * kind (0 row pass, 1 column pass, 2/3 weighted column passes A/B), the
+ * summed milliseconds and launch counts since the previous read-out. */
+int bpmh_plan_set_timing(bpmh_plan plan, int enable);
+int bpmh_plan_kernel_times(bpmh_plan plan, float* ms, int* counts);
+
+/* Fill H (ny, nx) complex64 with the angular-spectrum propagator.
+ * kind BPMH_H_F32 replaces the compute_propagator kernel launch
+ * (bpm/bpm/bpm_3d.py:192-196, bpm_3d_kernels.cl:408-432), all arguments used
+ * as float32; kind BPMH_H_F64 replaces the host numpy H of the class path
+ * (bpm/bpmclass/bpm3d.py:88-107) uploaded at bpm/bpmclass/_bpm3d_ocl.py:46. */
+int bpmh_propagator_table(int nx, int ny, int kind, double dx, double dy, double dz,
+                          double n0, double lam, float* H, void* stream);
+
+/* Run the whole split-step loop over nz planes (nz-1 steps) for `batch`
+ * fields that share the grid (e.g. illumination angles).  Replaces the loop
+ * body of bpm/bpm/bpm_3d.py:283-382 (fft, mult / mult_propagator /
+ * mult_propagator_buff, dn_sum/u_sum reductions, scattering reductions, ifft,
+ * mult_dn* kernels, copy_intens) and of bpm/bpmclass/_bpm3d_ocl.py:100-126.
+ *
+ *  plane      (batch, ny, nx) c64: u0 on entry, last plane u[nz-1] on exit
+ *  dn         (nz, ny, nx) of dn_kind per item, item b at dn + b*dn_batch_stride
+ *             elements (0: all items share one volume); NULL for BPMH_DN_NONE
+ *  mean_mode  BPMH_MEAN_*; IGNORE uses H (table from bpmh_propagator_table),
+ *             SIMPLE uses n_eff[z] (float32 n0+mean(dn[z])) and dn0[z],
+ *             WEIGHTED forms dn0 from dn slice 0 on device (bpm_3d.py:149-159)
+ *  n0,k0,dx,dy,dz  float32 kernel arguments of the on-the-fly propagator
+ *  unit_k     float32(k0*dz) of the phase screen
+ *  absorb     taper width in pixels (0: none)
+ *  out_field  (batch, nz, ny, nx) c64 or NULL     -- return_full, field
+ *  out_intens (batch, nz, ny, nx) f32 or NULL     -- return_full, intensity
+ *  out_sum    (batch, ny, nx) f32 or NULL: sum over all nz planes of |u|^2
+ *  scat_w,g_w (ny, nx) f32 cos(theta), cos(theta)^2; plain (nz) c64 the
+ *             Nx*Ny*exp(-i k0 n0 z dz) DC term; out_scat (batch, nz, 2) f32
+ *             receives the unnormalised sums (bpm_3d.py:227-260,335-341);
+ *             all NULL when scattering is not requested. */
+int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane, const void* dn,
+                   int dn_kind, long long dn_batch_stride, int mean_mode, const float* H,
+                   const float* n_eff, const float* dn0, float n0, float k0, float dx,
+                   float dy, float dz, float unit_k, int absorb, float* out_field,
+                   float* out_intens, float* out_sum, const float* scat_w, const float* g_w,
+                   const float* plain, float* out_scat, void* stream);
+
+/* Debye-Wolf kernel variants */
+#define BPMH_DW_INDEX 0  /* debye_wolf, bpm/psf/kernels/psf_debye.cl:24-113 */
+#define BPMH_DW_SIN 1    /* debye_wolf_plane / debye_wolf_at, psf_debye.cl:117-269 */
+#define BPMH_DW_GAUSS 2  /* debye_wolf_gauss, psf_debye.cl:273-362 */
+
+/* Vectorial Debye-Wolf field on a regular (nz, ny, nx) grid with coordinates
+ * x = x1 + i*(x2-x1)/(nx-1) (same for y, z).  Replaces the debye_wolf /
+ * debye_wolf_gauss launches of bpm/psf/_focus_fields_debye.py:102-109,
+ * :265-274, :404-413 (and debye_wolf_plane at :201-208 with nz=1, z1=z2).
+ * alphas: HOST array of n_alphas float32 aperture angles (annulus pairs);
+ * int_steps: the reference's compile-time INT_STEPS (runtime here).
+ * Outputs ex, ey, ez (c64) and I (f32) may be NULL individually.
+ * i_mode 0: I = |Ex|^2+|Ey|^2+|Ez|^2; 1: I = |Ex|^2 (debye_wolf_at). */
+int bpmh_debye_wolf(int nx, int ny, int nz, float x1, float x2, float y1, float y2, float z1,
+                    float z2, float lam, const float* alphas, int n_alphas, int int_steps,
+                    float ex0, float ey0, int variant, float sig, int i_mode, float* ex,
+                    float* ey, float* ez, float* I, void* stream);
+
+/* Same integrand at n arbitrary points (device arrays x, y, z).  Replaces
+ * debye_wolf_at, bpm/psf/_focus_fields_debye.py:346-352. */
+int bpmh_debye_wolf_points(int n, const float* x, const float* y, const float* z, float lam,
+                           const float* alphas, int n_alphas, int int_steps, float ex0,
+                           float ey0, int variant, float sig, int i_mode, float* ex,
+                           float* ey, float* ez, float* I, void* stream);
+
+/* Spread an (nz0/2+1, ny0/2+1, nx0/2+1) octant to the full (nz0, ny0, nx0)
+ * grid: index f -> |f - n0/2|, fields conjugated for f_z <= nz0/2.  Replaces
+ * the numpy mirror of bpm/psf/_focus_fields_debye.py:116-143.  Any of the
+ * four array pairs may be NULL. */
+int bpmh_debye_mirror(int nx0, int ny0, int nz0, const float* u_oct, const float* ex_oct,
+                      const float* ey_oct, const float* ez_oct, float* u, float* ex,
+                      float* ey, float* ez, void* stream);
+
+/* out[i] = J_n(x[i]) for n in {0,1,2} with this library's Bessel functions
+ * (device arrays).  Replaces the bessel_fill self-test kernel,
+ * bpm/psf/kernels/bessel.cl:120-129 (bpm/psf/_focus_fields_debye.py:34-42). */
+int bpmh_bessel_fill(int n, const float* x, float* out, int count, void* stream);
+
+/* Host (CPU) evaluation of the same Bessel code, for validating it without a
+ * GPU (double precision in and out; host arrays).  Not part of any hot path. */
+int bpmh_bessel_host(int n, const double* x, double* out, int count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BPM_HIP_H */

@@ -1,0 +1,76 @@
+"""C ABI of libbpmhip.so on the CPU: it loads, exports every function declared
+in include/bpm_hip.h, rejects bad arguments before touching a device, and its
+Bessel code (the one the Debye kernels run) matches scipy."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+from scipy.special import jv
+
+from bpm_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "bpm_hip.h")
+
+
+def header_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(bpmh_[a-z_0-9]+)\s*\(", txt)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail("libbpmhip.so is not built (run python -m bpm_amd.build)")
+    return _lib.load()
+
+
+def test_every_declared_symbol_is_exported_and_bound(lib):
+    names = header_functions()
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(lib, n), n
+        assert n in _lib.SIGNATURES, "no ctypes signature for " + n
+    assert lib.bpmh_version() == 100
+
+
+def test_plan_rejects_unsupported_sizes_without_gpu(lib):
+    h = ctypes.c_void_p()
+    for nx, ny in ((100, 64), (64, 8), (8192, 64), (0, 16)):
+        rc = lib.bpmh_plan_create(nx, ny, ctypes.byref(h))
+        assert rc == _lib.ERR_UNSUPPORTED
+        assert b"powers of two" in lib.bpmh_last_error()
+    with pytest.raises(NotImplementedError):
+        _lib.check(lib.bpmh_plan_create(48, 64, ctypes.byref(h)))
+
+
+def test_argument_validation_paths(lib):
+    # invalid before any launch: null plan / bad Debye arguments / bad Bessel order
+    rc = lib.bpmh_propagate(None, 1, 2, None, None, 0, 0, 0, None, None, None, 1., 1., .1, .1,
+                            .1, 1., 0, None, None, None, None, None, None, None, None)
+    assert rc == _lib.ERR_INVALID
+    al = np.array([0., .5, .6], np.float32)
+    rc = lib.bpmh_debye_wolf(8, 8, 8, 0, 1, 0, 1, 0, 1, .5, al.ctypes.data, 3, 100, 1., 0., 0,
+                             0., 0, None, None, None, None, None)
+    assert rc == _lib.ERR_INVALID
+    rc = lib.bpmh_debye_mirror(9, 8, 8, None, None, None, None, None, None, None, None, None)
+    assert rc == _lib.ERR_UNSUPPORTED
+    rc = lib.bpmh_bessel_fill(3, None, None, 4, None)
+    assert rc == _lib.ERR_UNSUPPORTED
+    with pytest.raises(ValueError):
+        _lib.check(_lib.ERR_INVALID)
+
+
+def test_bessel_code_matches_scipy_on_host(lib):
+    x = np.concatenate([np.linspace(0, 140, 140001), np.linspace(13.99, 14.01, 101),
+                        [1e-20, 1e-6, -3.5]])
+    for n in range(3):
+        out = np.empty_like(x)
+        assert lib.bpmh_bessel_host(n, x.ctypes.data, out.ctypes.data, x.size) == 0
+        ref = jv(n, x)
+        assert np.abs(out - ref).max() < 1e-10
+        small = (np.abs(x) < 1) & (x != 0)
+        assert np.allclose(out[small], ref[small], rtol=1e-13, atol=0)

@@ -1,0 +1,212 @@
+"""GPU parity of the HIP split-step BPM (through the C ABI) against the oracle.
+
+Bar: complex64 fields and intensities within relative
+L2 <= 1e-4 of the complex64 restatement of the reference float32 path; the
+error against the float64 oracle is printed alongside.  Full-size cases are
+checked through size-independent properties (closed form, Parseval,
+batch == independent runs).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import bpm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+LAM, N0 = .5, 1.33
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def bpm():
+    import bpm_amd
+    from bpm_amd import _lib
+    _lib.torch()  # raises without a GPU or without libbpmhip.so
+    return bpm_amd
+
+
+def rng_dn(shape, kind, seed=0, amp=.05):
+    rng = np.random.default_rng(seed)
+    Nz, Ny, Nx = shape
+    from bpm_amd.workloads import random_spheres, spheres_dn
+    size = (Nx, Ny, Nz)
+    c, r, d = random_spheres(rng, size, (.1, .1, .1), 12, (.3, 1.), (.01, amp))
+    dn = spheres_dn(size, (.1, .1, .1), c, r, d)
+    if kind == "c64":
+        dn = (dn - 1j * 0.2 * dn).astype(np.complex64)
+    return dn
+
+
+@pytest.mark.parametrize("mean_mode", ["ignore", "simple", "weighted"])
+@pytest.mark.parametrize("kind", ["none", "f32", "f16", "c64"])
+@pytest.mark.parametrize("absorb", [0, 6])
+def test_bpm_3d_modes_vs_oracle(bpm, mean_mode, kind, absorb):
+    Nx, Ny, Nz = 64, 32, 24
+    say("case bpm_3d", mean_mode, kind, absorb, (Nx, Ny, Nz))
+    dn = None if kind == "none" else rng_dn((Nz, Ny, Nx), kind, seed=1)
+    half = kind == "f16"
+    rng = np.random.default_rng(2)
+    u0 = np.exp(1j * rng.uniform(0, .5, (Ny, Nx))).astype(np.complex64)
+    got = bpm.bpm_3d((Nx, Ny, Nz), (.1, .1, .1), LAM, u0=u0, dn=dn, n0=N0, mean_mode=mean_mode,
+                     absorbing_width=absorb, store_dn_as_half=half)
+    ref = O.bpm_3d((Nx, Ny, Nz), (.1, .1, .1), LAM, u0, dn, N0, mean_mode=mean_mode,
+                   absorbing_width=absorb, store_dn_as_half=half)
+    e = O.rel_l2(got, ref)
+    say("   rel-L2 vs c64 oracle %.2e" % e)
+    assert got.shape == (Nz, Ny, Nx) and got.dtype == np.complex64
+    assert e < TOL
+
+
+@pytest.mark.parametrize("nx,ny", [(16, 16), (32, 64), (128, 16), (256, 256), (512, 128),
+                                   (1024, 64), (2048, 32), (64, 2048), (4096, 16), (16, 4096)])
+def test_all_grid_sizes(bpm, nx, ny):
+    Nz = 5
+    say("case sizes", nx, ny)
+    dn = rng_dn((Nz, ny, nx), "f32", seed=nx + ny)
+    rng = np.random.default_rng(3)
+    u0 = np.exp(1j * rng.uniform(0, 1, (ny, nx))).astype(np.complex64)
+    got = bpm.bpm_3d((nx, ny, Nz), (.1, .1, .1), LAM, u0=u0, dn=dn, n0=N0, mean_mode="weighted",
+                     return_full=False)
+    ref = O.bpm_3d((nx, ny, Nz), (.1, .1, .1), LAM, u0, dn, N0, mean_mode="weighted",
+                   return_full=False)
+    e = O.rel_l2(got, ref)
+    say("   rel-L2 %.2e" % e)
+    assert e < TOL
+
+
+def test_outputs_intensity_scattering_split(bpm):
+    Nx, Ny, Nz = 64, 64, 21
+    say("case outputs")
+    dn = rng_dn((Nz, Ny, Nx), "f32", seed=4)
+    kw = dict(n0=N0, mean_mode="ignore")
+    # intensity volume, last plane field / intensity
+    ui = bpm.bpm_3d((Nx, Ny, Nz), (.1, .1, .1), LAM, dn=dn, return_field=False, **kw)
+    ri = O.bpm_3d((Nx, Ny, Nz), (.1, .1, .1), LAM, None, dn, return_field=False, **kw)
+    assert ui.dtype == np.float32 and O.rel_l2(ui, ri) < TOL
+    ul = bpm.bpm_3d((Nx, Ny, Nz), (.1, .1, .1), LAM, dn=dn, return_full=False, **kw)
+    rl = O.bpm_3d((Nx, Ny, Nz), (.1, .1, .1), LAM, None, dn, return_full=False, **kw)
+    assert ul.shape == (Ny, Nx) and O.rel_l2(ul, rl) < TOL
+    # scattering cross-section and g factor
+    u, p, g = bpm.bpm_3d((Nx, Ny, Nz), (.1, .1, .1), LAM, dn=dn, return_scattering=True,
+                         return_g=True, **kw)
+    ru, rp, rg = O.bpm_3d((Nx, Ny, Nz), (.1, .1, .1), LAM, None, dn, return_scattering=True,
+                          return_g=True, **kw)
+    say("   p rel %.2e g rel %.2e" % (O.rel_l2(p[1:], rp[1:]), O.rel_l2(g[1:], rg[1:])))
+    assert p[0] == 0 and np.isnan(g[0])
+    assert O.rel_l2(p[1:], rp[1:]) < 1e-3 and O.rel_l2(g[1:], rg[1:]) < 1e-3
+    # z-chunked, weighted, with scattering
+    for nv in (2, 3):
+        us, ps = bpm.bpm_3d((Nx, Ny, Nz), (.1, .1, .1), LAM, dn=dn, n_volumes=nv,
+                            return_scattering=True, n0=N0)
+        rs, rps = O.bpm_3d_split((Nx, Ny, Nz), (.1, .1, .1), LAM, None, dn, nv, N0,
+                                 return_scattering=True)
+        assert O.rel_l2(us, rs) < TOL and O.rel_l2(ps[1:], rps[1:]) < 1e-3
+
+
+def test_field_intensity_identity_reference_case(bpm):
+    # reference tests/test_field_intens.py:14-43 (its only assert), Nz reduced
+    Nx, Ny, Nz = 256, 256, 128
+    say("case field==intensity")
+    dx, dz = .05, .05
+    x = dx * np.arange(-Nx / 2, Nx / 2)
+    z = dz * np.arange(-Nz / 2, Nz / 2)
+    Z, Y, X = np.meshgrid(z, x, x, indexing="ij")
+    dn = .05 * (np.sqrt(X ** 2 + Y ** 2 + Z ** 2) < 1.)
+    u1 = bpm.bpm_3d((Nx, Ny, Nz), (dx, dx, dz), LAM, dn=dn, n_volumes=2, return_field=True)
+    u2 = bpm.bpm_3d((Nx, Ny, Nz), (dx, dx, dz), LAM, dn=dn, n_volumes=2, return_field=False)
+    assert np.allclose(np.abs(u1) ** 2, u2)
+
+
+def test_class_path_vs_oracle(bpm):
+    Nx, Ny, Nz = 128, 64, 33
+    say("case Bpm3d")
+    for kind in ("f32", "c64"):
+        dn = rng_dn((Nz, Ny, Nx), kind, seed=5)
+        m = bpm.Bpm3d((Nx, Ny, Nz), (.1, .1, .1), lam=LAM, n0=N0, dn=dn)
+        for absorb in (0, 4):
+            got = m._propagate_single(absorbing_width=absorb)
+            ref = O.propagate_single((Nx, Ny, Nz), (.1, .1, .1), LAM, N0, dn, None,
+                                     absorbing_width=absorb)
+            e = O.rel_l2(got, ref)
+            say("   %s absorb=%d rel-L2 %.2e" % (kind, absorb, e))
+            assert e < TOL
+        gi = m.propagate(return_full=False, return_intensity=True)
+        assert gi.shape == (Ny, Nx) and O.rel_l2(gi, np.abs(ref[-1]) ** 2) < TOL
+    with pytest.raises(ValueError):
+        m.set_dn(np.zeros((Nz, Ny, Nx + 1), np.float32))
+    with pytest.raises(NotImplementedError):
+        bpm.Bpm3d((Nx, Ny, Nz), (.1, .1, .1), backend="cuda")
+
+
+def test_errors_map_to_reference_exceptions(bpm):
+    with pytest.raises(NotImplementedError):
+        bpm.bpm_3d((32, 32, 4), (.1, .1, .1), mean_mode="median")
+    with pytest.raises(NotImplementedError):
+        bpm.bpm_3d((32, 32, 4), (.1, .1, .1), subsample=2)
+    with pytest.raises(NotImplementedError):
+        bpm.bpm_3d((48, 32, 4), (.1, .1, .1))
+    with pytest.raises(ValueError):
+        bpm.bpm_3d((32, 32, 4), (.1, .1, .1), dn=np.zeros((4, 32, 16), np.float32))
+
+
+def test_full_size_plane_wave_closed_form(bpm):
+    # 1024^2 grid, 256 steps through a layered medium: exact solution known
+    Nx = Ny = 1024
+    Nz, dx, dz = 256, .1, .1
+    say("case 1024^2 plane wave")
+    mx, my = 7, -3
+    kx = 2 * np.pi * mx / (Nx * dx)
+    ky = 2 * np.pi * my / (Ny * dx)
+    x = dx * np.arange(Nx)
+    u0 = np.exp(-1j * (kx * x[None, :] + ky * x[:, None])).astype(np.complex64)
+    c = (0.03 * np.sin(np.arange(Nz) / 17.)).astype(np.float32)
+    dn = np.broadcast_to(c[:, None, None], (Nz, Ny, Nx)).copy()
+    got = bpm.bpm_3d((Nx, Ny, Nz), (dx, dx, dz), LAM, u0=u0, dn=dn, n0=N0, mean_mode="ignore",
+                     return_full=False)
+    kz = np.sqrt(N0 ** 2 * (2 * np.pi / LAM) ** 2 - kx ** 2 - ky ** 2)
+    ph = (Nz - 1) * dz * kz + 2 * np.pi / LAM * dz * c[1:].astype(np.float64).sum()
+    exact = u0.astype(np.complex128) * np.exp(-1j * ph)
+    e = O.rel_l2(got, exact)
+    say("   rel-L2 vs closed form %.2e" % e)
+    assert e < 1e-4
+
+
+def test_full_size_energy_conservation_and_batch(bpm):
+    # dx = 0.3 um: every FFT frequency propagates, so ||u_z|| is conserved.
+    import torch
+    from bpm_amd import bpm3d as B
+    from bpm_amd import _lib
+    Nx = Ny = 1024
+    Nz, B_ = 64, 3
+    say("case 1024^2 energy + batch")
+    rng = np.random.default_rng(11)
+    dn = torch.from_numpy(rng.uniform(-.05, .05, (Nz, Ny, Nx)).astype(np.float32)).cuda()
+    u0 = np.exp(1j * rng.uniform(0, 6, (B_, Ny, Nx))).astype(np.complex64)
+    units = (.3, .3, .1)
+    H = B.propagator_table(Nx, Ny, units, LAM, N0)
+    plane = torch.from_numpy(u0).cuda()
+    acc = torch.empty((B_, Ny, Nx), dtype=torch.float32, device="cuda")
+    B.propagate_fields(plane, Nz, units, LAM, N0, dn=dn, dn_kind=_lib.DN_F32, mean_mode="ignore",
+                       H=H, out_sum=acc)
+    out = plane.cpu().numpy()
+    e0 = np.sum(np.abs(u0.astype(np.complex128)) ** 2, axis=(1, 2))
+    e1 = np.sum(np.abs(out.astype(np.complex128)) ** 2, axis=(1, 2))
+    drift = np.abs(e1 / e0 - 1).max()
+    say("   energy drift over %d steps: %.2e" % (Nz - 1, drift))
+    assert drift < 1e-5
+    # summed intensity of a unitary propagation: total = Nz * energy
+    tot = acc.double().sum(dim=(1, 2)).cpu().numpy()
+    assert np.allclose(tot, Nz * e0, rtol=1e-5)
+    # each batch item equals its own single run, bit for bit
+    for b in range(B_):
+        p1 = torch.from_numpy(u0[b:b + 1]).cuda()
+        B.propagate_fields(p1, Nz, units, LAM, N0, dn=dn, dn_kind=_lib.DN_F32,
+                           mean_mode="ignore", H=H)
+        assert torch.equal(p1[0], plane[b])

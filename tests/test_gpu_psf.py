@@ -1,0 +1,117 @@
+"""GPU parity of the HIP Debye-Wolf kernels (through the C ABI) against the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import bpm_oracle as O
+from oracle import psf_oracle as P
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def bpm():
+    import bpm_amd
+    from bpm_amd import _lib
+    _lib.torch()
+    return bpm_amd
+
+
+@pytest.mark.parametrize("NA,n0,steps", [(1.1, 1.33, 200), ([.3, .5, .7, .9], 1., 64), (.4, 1., 100)])
+def test_focus_field_debye_vs_oracle(bpm, NA, n0, steps):
+    shape, units, lam = (32, 24, 40), (.05, .06, .05), .5
+    say("case focus_field_debye", NA, n0, steps)
+    got = bpm.focus_field_debye(shape, units, lam, NA, n0=n0, n_integration_steps=steps)
+    ref = P.focus_field_debye(shape, units, lam, NA, n0=n0, n_integration_steps=steps)
+    ref64 = P.focus_field_debye(shape, units, lam, NA, n0=n0, n_integration_steps=steps,
+                                precision="f64")
+    for name, g, r, r64 in zip(("u", "ex", "ey", "ez"), got, ref, ref64):
+        e, e64 = O.rel_l2(g, r), O.rel_l2(g, r64)
+        say("   %s rel-L2 vs f32 oracle %.2e, vs f64 %.2e" % (name, e, e64))
+        assert g.shape == (shape[2], shape[1], shape[0])
+        assert e < TOL
+
+
+def test_psf_and_psf_u0(bpm):
+    say("case psf, psf_u0, lightsheet")
+    u = bpm.psf((16, 16, 16), (.1, .1, .1), .5, .8)
+    r = P.focus_field_debye((16, 16, 16), (.1, .1, .1), .5, .8)[0]
+    assert O.rel_l2(u, r) < TOL
+    u0 = bpm.psf_u0((64, 48), (.1, .1), zfoc=3., NA=.4, lam=.5, n0=1.33)
+    r0 = P.focus_field_debye((64, 48, 4), (.1, .1, 1.5), .5, .4, n0=1.33)[1][0]
+    assert u0.shape == (48, 64) and O.rel_l2(u0, r0) < TOL
+    ls = bpm.psf_lightsheet((16, 16, 16), (.1, .1, .1), .4, .3, .5, .8)
+    a = P.focus_field_debye((16, 16, 16), (.1, .1, .1), .5, .8)[0]
+    b = P.focus_field_debye((16, 16, 16), (.1, .1, .1), .4, .3)[0].transpose((2, 1, 0))
+    assert O.rel_l2(ls, a * b) < TOL
+    with pytest.raises(NotImplementedError):
+        bpm.psf((15, 16, 16), (.1, .1, .1), .5, .8)
+
+
+def test_secondary_variants(bpm):
+    say("case plane / at / gauss / debye3")
+    ex = bpm.focus_field_debye_plane((40, 32), (.1, .1), z=-1.3, lam=.5, NA=.7, n0=1.2,
+                                     n_integration_steps=80)
+    rx = P.focus_field_debye_plane((40, 32), (.1, .1), z=-1.3, lam=.5, NA=.7, n0=1.2,
+                                   n_integration_steps=80)
+    assert O.rel_l2(ex, rx) < TOL
+    rng = np.random.default_rng(0)
+    x, y, z = (rng.uniform(-1, 1, (5, 7)) for _ in range(3))
+    got = bpm.focus_field_debye_at(x, y, z, .5, .9, n0=1.1, n_integration_steps=90)
+    ref = P.focus_field_debye_at(x, y, z, .5, .9, n0=1.1, n_integration_steps=90)
+    for g, r in zip(got, ref):
+        assert g.shape == (5, 7) and O.rel_l2(g, r) < TOL
+    got = bpm.focus_field_debye_gauss((20, 18, 16), (.1, .1, .1), .5, [0., .8], sig=.6,
+                                      n_integration_steps=60)
+    ref = P.focus_field_debye_gauss((20, 18, 16), (.1, .1, .1), .5, [0., .8], sig=.6,
+                                    n_integration_steps=60)
+    for g, r in zip(got, ref):
+        assert O.rel_l2(g, r) < TOL
+    got = bpm.focus_field_debye3((17, 16, 15), (.1, .1, .1), .5, .7, n_integration_steps=60)
+    ref = P.focus_field_debye3((17, 16, 15), (.1, .1, .1), .5, .7, n_integration_steps=60)
+    for g, r in zip(got, ref):
+        assert O.rel_l2(g, r) < TOL
+
+
+def test_bessel_fill_vs_scipy(bpm):
+    from scipy.special import jv
+    x = np.linspace(0, 130, 100001).astype(np.float32)
+    for n in range(3):
+        got = bpm.psf.bessel_jn(n, x)
+        ref = jv(n, x.astype(np.float64))
+        assert np.abs(got - ref).max() < 1e-7
+
+
+def test_full_size_debye_planes(bpm):
+    # the 256^3, NA 1.1, n0 1.33, 512-step workload; planes of the octant
+    # checked against the oracle evaluated on those planes only
+    from bpm_amd import psf as Q
+    shape, units, lam, NA, n0, S = (256, 256, 256), (.05,) * 3, .5, 1.1, 1.33, 512
+    say("case 256^3 Debye")
+    u, ex, ey, ez = (t.cpu().numpy() for t in Q.focus_field_debye_device(shape, units, lam, NA, n0, S))
+    N = 129
+    a = P._alphas(NA, n0)
+    zmax = np.float32(.05 * (N - 1.))
+    for k in (0, 37, 128):
+        zk = np.float32(0) + (np.float32(k) * (zmax - np.float32(0))) / np.float32(N - 1.)
+        rex, rey, rez, ru = P.debye_wolf_grid(N, N, 1, 0., .05 * (N - 1.), 0., .05 * (N - 1.), zk, zk,
+                                              lam / n0, a, S)
+        # full-grid plane of octant plane k: 128+k for z > 0; plane 128 (z=0)
+        # and plane 0 (z=-6.4, octant 128) are conjugated
+        fz = 128 + k if 0 < k < 128 else (128 if k == 0 else 0)
+        if k in (0, 128):
+            rex, rey, rez = np.conj(rex), np.conj(rey), np.conj(rez)
+        q = (0, slice(0, 128), slice(0, 128))
+        for g, r in ((u[fz, 128:, 128:], ru[q]), (ex[fz, 128:, 128:], rex[q]),
+                     (ey[fz, 128:, 128:], rey[q]), (ez[fz, 128:, 128:], rez[q])):
+            e = O.rel_l2(g, r)
+            say("   plane %d rel-L2 %.2e" % (k, e))
+            assert e < TOL
