@@ -64,7 +64,7 @@ def cpu_baseline(u0, dn, units, lam, n0, planes):
 
 def debye_extra(torch):
     """The 256^3 Debye-Wolf workload (NA 1.1, n0 1.33, 512 theta steps)."""
-    from bpm_amd import psf as Q
+    from bpm_amd import focus_fields as Q
     from bpm_amd import _lib
     shape, units, lam, NA, n0, S = (256, 256, 256), (.05,) * 3, .5, 1.1, 1.33, 512
     Q.focus_field_debye_device(shape, units, lam, NA, n0, S)

@@ -10,7 +10,7 @@ as in the reference; torch GPU tensors are accepted as inputs.
 """
 from .bpm3d import bpm_3d
 from .bpmclass import Bpm3d
-from .psf import (psf, psf_u0, psf_lightsheet, focus_field_debye, focus_field_debye_plane,
+from .focus_fields import (psf, psf_u0, psf_lightsheet, focus_field_debye, focus_field_debye_plane,
                   focus_field_debye_at, focus_field_debye_gauss, focus_field_debye3)
 
 __all__ = ["bpm_3d", "Bpm3d", "psf", "psf_u0", "psf_lightsheet", "focus_field_debye",

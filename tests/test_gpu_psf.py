@@ -83,9 +83,10 @@ def test_secondary_variants(bpm):
 
 def test_bessel_fill_vs_scipy(bpm):
     from scipy.special import jv
+    from bpm_amd import focus_fields
     x = np.linspace(0, 130, 100001).astype(np.float32)
     for n in range(3):
-        got = bpm.psf.bessel_jn(n, x)
+        got = focus_fields.bessel_jn(n, x)
         ref = jv(n, x.astype(np.float64))
         assert np.abs(got - ref).max() < 1e-7
 
@@ -93,7 +94,7 @@ def test_bessel_fill_vs_scipy(bpm):
 def test_full_size_debye_planes(bpm):
     # the 256^3, NA 1.1, n0 1.33, 512-step workload; planes of the octant
     # checked against the oracle evaluated on those planes only
-    from bpm_amd import psf as Q
+    from bpm_amd import focus_fields as Q
     shape, units, lam, NA, n0, S = (256, 256, 256), (.05,) * 3, .5, 1.1, 1.33, 512
     say("case 256^3 Debye")
     u, ex, ey, ez = (t.cpu().numpy() for t in Q.focus_field_debye_device(shape, units, lam, NA, n0, S))
