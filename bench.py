@@ -44,6 +44,8 @@ def parse():
                     help="planes of the workload the CPU baseline propagates")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-debye", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the z-loop kernel by kernel instead of replaying a HIP graph")
     return ap.parse_args()
 
 
@@ -116,17 +118,26 @@ def main():
         B.propagate_fields(plane, Nz, units, lam, n0, dn=dn_g, dn_kind=_lib.DN_F32,
                            mean_mode="ignore", H=H, out_sum=acc)
 
+    _lib.check(lib.bpmh_plan_set_graph(plan, 0 if a.no_graph else 1))
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
 
-    _lib.check(lib.bpmh_plan_set_timing(plan, 1))
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
         step()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+
+    # per-kernel durations: the same steps again, eagerly launched with a HIP
+    # event pair around every kernel (on the stream the kernels run on)
+    _lib.check(lib.bpmh_plan_set_timing(plan, 1))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    el_events = time.perf_counter() - t1
     ms = (ctypes.c_float * 4)()
     cnt = (ctypes.c_int * 4)()
     _lib.check(lib.bpmh_plan_kernel_times(plan, ms, cnt))
@@ -170,6 +181,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None},
         "kernels": kern,
+        "launch": "eager" if a.no_graph else "hip-graph",
+        "ms_per_step_eager_with_events": el_events / a.steps * 1e3,
     }
     if not a.no_debye:
         out["debye"] = debye_extra(torch)

@@ -310,10 +310,15 @@ k_col(ColArgs a) {
       an = n_term_f32(n, a.k0);
       kx2 = kcomp2_f32(x, a.nx, a.dx);
     }
+    // H depends on kx^2 and ky^2 only, so H[j][i] = H[(ny-j)%ny][(nx-i)%nx]
+    // holds bit for bit; reading the quadrant j<=ny/2, i<=nx/2 keeps the
+    // table's footprint at a quarter (L2-resident per XCD strip).
+    const int qx = min(x, a.nx - x);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int ky = t + m * T;
-      const float2 h = (a.hmode == 0) ? a.H[(size_t)ky * a.nx + x]
+      const int qy = min(ky, NY - ky);
+      const float2 h = (a.hmode == 0) ? a.H[(size_t)qy * a.nx + qx]
                                       : prop_f32(kx2, kcomp2_f32(ky, NY, a.dy), an, a.dz);
       v[m] = c_mul(v[m], h);
     }
@@ -408,6 +413,12 @@ struct bpmh_plan_s {
   std::vector<hipEvent_t> ev;   // pairs (start, stop)
   std::vector<int> ev_kind;     // kind of each pair
   size_t ev_used;               // pairs recorded since the last read-out
+  // graph mode: the launch sequence of bpmh_propagate is captured once per
+  // distinct argument list and replayed with hipGraphLaunch
+  int graph_mode;
+  hipStream_t cap_stream;
+  hipGraphExec_t gexec;
+  std::vector<unsigned char> gkey;
 };
 
 static bool pow2_in_range(int n) { return n >= 16 && n <= 4096 && (n & (n - 1)) == 0; }
@@ -473,6 +484,9 @@ extern "C" int bpmh_plan_create(int nx, int ny, bpmh_plan* out) {
   p->scratch_bytes = 0;
   p->timing = 0;
   p->ev_used = 0;
+  p->graph_mode = 0;
+  p->cap_stream = nullptr;
+  p->gexec = nullptr;
   (void)hipGetDevice(&p->device);
   std::vector<float2> tx, ty;
   make_twiddles(nx, tx);
@@ -499,7 +513,20 @@ extern "C" int bpmh_plan_destroy(bpmh_plan p) {
   (void)hipFree(p->tw_y);
   if (p->scratch) (void)hipFree(p->scratch);
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
   delete p;
+  return BPMH_OK;
+}
+
+extern "C" int bpmh_plan_set_graph(bpmh_plan p, int enable) {
+  if (!p) return bpmh_set_error(BPMH_ERR_INVALID, "null plan");
+  p->graph_mode = enable != 0;
+  if (!p->graph_mode && p->gexec) {
+    HIP_TRY(hipGraphExecDestroy(p->gexec));
+    p->gexec = nullptr;
+    p->gkey.clear();
+  }
   return BPMH_OK;
 }
 
@@ -723,6 +750,32 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, 
     rc = timing_reserve(plan, (size_t)nz * 3 + 1);
     if (rc) return rc;
   }
+  const bool use_graph = plan->graph_mode && !plan->timing;
+  std::vector<unsigned char> key;
+  if (use_graph) {
+    auto put = [&key](const void* v, size_t n) {
+      const unsigned char* c = (const unsigned char*)v;
+      key.insert(key.end(), c, c + n);
+    };
+    put(&ra, sizeof ra);
+    put(&ca, sizeof ca);
+    put(&batch, sizeof batch);
+    put(&nz, sizeof nz);
+    put(&mean_mode, sizeof mean_mode);
+    put(&out_scat, sizeof out_scat);
+    if (plan->gexec && key == plan->gkey) {
+      HIP_TRY(hipGraphLaunch(plan->gexec, s));
+      return BPMH_OK;
+    }
+    if (!plan->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&plan->cap_stream, hipStreamNonBlocking));
+    if (plan->gexec) {
+      HIP_TRY(hipGraphExecDestroy(plan->gexec));
+      plan->gexec = nullptr;
+      plan->gkey.clear();
+    }
+    HIP_TRY(hipStreamBeginCapture(plan->cap_stream, hipStreamCaptureModeThreadLocal));
+    s = plan->cap_stream;
+  }
   auto timed = [&](int kind, auto&& launch) {
     if (!plan->timing) {
       launch();
@@ -760,6 +813,24 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, 
   if (out_scat) {
     hipLaunchKernelGGL(k_scatter_final, dim3((nz + 63) / 64, batch), dim3(64), 0, s, spart,
                        out_scat, nz, ntiles);
+  }
+  if (use_graph) {
+    hipGraph_t g = nullptr;
+    hipError_t le = hipGetLastError();
+    hipError_t ce = hipStreamEndCapture(plan->cap_stream, &g);
+    if (le != hipSuccess || ce != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    hipError_t ie = hipGraphInstantiate(&plan->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+      plan->gexec = nullptr;
+      return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(ie));
+    }
+    plan->gkey = key;
+    HIP_TRY(hipGraphLaunch(plan->gexec, (hipStream_t)stream));
+    return BPMH_OK;
   }
   HIP_TRY(hipGetLastError());
   return BPMH_OK;

@@ -65,6 +65,13 @@ int bpmh_plan_destroy(bpmh_plan plan);
 int bpmh_plan_set_timing(bpmh_plan plan, int enable);
 int bpmh_plan_kernel_times(bpmh_plan plan, float* ms, int* counts);
 
+/* Graph mode: bpmh_propagate captures its whole launch sequence (2 or 3
+ * kernels per z-step) into a HIP graph the first time it sees an argument
+ * list and replays the graph (one hipGraphLaunch) while the arguments stay
+ * identical.  Replaces the per-kernel Python->pyopencl enqueue of the
+ * reference loop (bpm/bpm/bpm_3d.py:283-382).  Ignored while timing is on. */
+int bpmh_plan_set_graph(bpmh_plan plan, int enable);
+
 /* Fill H (ny, nx) complex64 with the angular-spectrum propagator.
  * kind BPMH_H_F32 replaces the compute_propagator kernel launch
  * (bpm/bpm/bpm_3d.py:192-196, bpm_3d_kernels.cl:408-432), all arguments used
@@ -82,7 +89,9 @@ int bpmh_propagator_table(int nx, int ny, int kind, double dx, double dy, double
  *  plane      (batch, ny, nx) c64: u0 on entry, last plane u[nz-1] on exit
  *  dn         (nz, ny, nx) of dn_kind per item, item b at dn + b*dn_batch_stride
  *             elements (0: all items share one volume); NULL for BPMH_DN_NONE
- *  mean_mode  BPMH_MEAN_*; IGNORE uses H (table from bpmh_propagator_table),
+ *  mean_mode  BPMH_MEAN_*; IGNORE uses H (a table from bpmh_propagator_table;
+ *             only its quadrant j <= ny/2, i <= nx/2 is read, the rest
+ *             follows from H[j][i] = H[(ny-j)%ny][(nx-i)%nx]),
  *             SIMPLE uses n_eff[z] (float32 n0+mean(dn[z])) and dn0[z],
  *             WEIGHTED forms dn0 from dn slice 0 on device (bpm_3d.py:149-159)
  *  n0,k0,dx,dy,dz  float32 kernel arguments of the on-the-fly propagator

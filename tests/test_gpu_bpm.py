@@ -210,3 +210,41 @@ def test_full_size_energy_conservation_and_batch(bpm):
         B.propagate_fields(p1, Nz, units, LAM, N0, dn=dn, dn_kind=_lib.DN_F32,
                            mean_mode="ignore", H=H)
         assert torch.equal(p1[0], plane[b])
+
+
+def test_graph_replay_matches_eager(bpm):
+    import torch
+    from bpm_amd import bpm3d as B
+    from bpm_amd import _lib
+    Nx, Ny, Nz = 256, 128, 40
+    say("case graph replay")
+    rng = np.random.default_rng(12)
+    dn = torch.from_numpy(rng.uniform(0, .05, (Nz, Ny, Nx)).astype(np.float32)).cuda()
+    u0 = torch.from_numpy(np.exp(1j * rng.uniform(0, 3, (1, Ny, Nx))).astype(np.complex64)).cuda()
+    units = (.1, .1, .1)
+    H = B.propagator_table(Nx, Ny, units, LAM, N0, _lib.H_F64)
+    plan = B.get_plan(Nx, Ny)
+    lib = _lib.load()
+    plane = torch.empty_like(u0)
+    acc = torch.empty((1, Ny, Nx), dtype=torch.float32, device="cuda")
+
+    def run(mode):
+        plane.copy_(u0)
+        B.propagate_fields(plane, Nz, units, LAM, N0, dn=dn, dn_kind=_lib.DN_F32,
+                           mean_mode=mode, H=H, out_sum=acc)
+        torch.cuda.synchronize()
+        return plane.clone(), acc.clone()
+
+    for mode in ("ignore", "weighted"):
+        _lib.check(lib.bpmh_plan_set_graph(plan, 0))
+        ref_p, ref_a = run(mode)
+        _lib.check(lib.bpmh_plan_set_graph(plan, 1))
+        try:
+            for r in range(4):
+                got_p, got_a = run(mode)
+                assert torch.equal(got_p, ref_p) and torch.equal(got_a, ref_a), (mode, r)
+                # unrelated eager work between replays
+                x = torch.randn(1 << 20, device="cuda")
+                (x * 2).sum().item()
+        finally:
+            _lib.check(lib.bpmh_plan_set_graph(plan, 0))
