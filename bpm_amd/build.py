@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbpmhip.so")
-SOURCES = ["bpm_kernels.hip", "debye.hip"]
+SOURCES = ["bpm_kernels.hip", "debye.hip", "spheres.hip"]
 ARCH = os.environ.get("BPMH_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

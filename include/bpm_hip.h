@@ -111,6 +111,24 @@ int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane, const void* 
                    float* out_intens, float* out_sum, const float* scat_w, const float* g_w,
                    const float* plain, float* out_scat, void* stream);
 
+/* Coated-sphere refractive-index volume (nz, ny, nx) float32: for each voxel
+ * at (i dx, j dy, k dz), dmin = distance to the nearest of the n_points
+ * centres (device float32 array n x 3, xyz, sorted by z) whose z is within
+ * rad2; dn = dmin < rad1 ? dn1 : dmin < rad2 ? dn2 : 0.  Replaces the fill_dn
+ * kernel (bpm/bpm/kernels/bpm_3d_spheres.cl:31-74) launched by
+ * create_dn_buffer (bpm/bpm/bpm_3d_spheres.py:11-38). */
+int bpmh_fill_dn_coated(int nx, int ny, int nz, float dx, float dy, float dz,
+                        const float* points, int n_points, float dn1, float rad1, float dn2,
+                        float rad2, float* dn, void* stream);
+
+/* Summed-sphere volume: dn = sum_s dn_s [|r - c_s| < r_s] with spheres a
+ * device float64 array n x 5 (cx, cy, cz, r, dn) in micrometres; distances
+ * in float64, float32 sum in sphere order.  No reference counterpart (the
+ * reference builds such volumes on the host); it generates the benchmark
+ * workloads' dn on the device instead of uploading 4 GiB. */
+int bpmh_spheres_dn(int nx, int ny, int nz, double dx, double dy, double dz,
+                    const double* spheres, int n_spheres, float* dn, void* stream);
+
 /* Debye-Wolf kernel variants */
 #define BPMH_DW_INDEX 0  /* debye_wolf, bpm/psf/kernels/psf_debye.cl:24-113 */
 #define BPMH_DW_SIN 1    /* debye_wolf_plane / debye_wolf_at, psf_debye.cl:117-269 */

@@ -47,6 +47,8 @@ bpm_3d_split       _bpm_3d_split  bpm/bpm/bpm_3d.py:863-992
 propagate_single   _Bpm3d_OCL._propagate_single  bpm/bpmclass/_bpm3d_ocl.py:66-134
                    pinned: same plane-wave / energy closed forms; equals
                    bpm_3d(mean_mode="ignore") up to the H variant.
+fill_dn_coated     fill_dn  bpm/bpm/kernels/bpm_3d_spheres.cl:31-74
+                   pinned: brute-force nearest centre over all points.
 """
 import numpy as np
 
@@ -56,7 +58,7 @@ PI_F32 = F32(3.14159265358979)
 
 __all__ = ["fftfreq_f32", "propagator_f32", "propagator_f64", "taper_f32",
            "screen", "bpm_3d", "bpm_3d_split", "propagate_single",
-           "intensity_f32", "rel_l2"]
+           "intensity_f32", "rel_l2", "fill_dn_coated"]
 
 
 def rel_l2(a, b):
@@ -476,3 +478,37 @@ def propagate_single(size, units, lam=.5, n0=1., dn=None, u0=None,
     if return_full:
         return out
     return inten(plane) if return_intensity else plane
+
+
+def fill_dn_coated(size, units, points, dn_inner=.0, rad_inner=0., dn_outer=.1, rad_outer=.4):
+    """Coated-sphere volume of fill_dn (bpm/bpm/kernels/bpm_3d_spheres.cl:31-74)
+    with create_dn_buffer's z-sort (bpm/bpm/bpm_3d_spheres.py:24-26), float32.
+
+    Pinned (tests/test_oracle_bpm.py): equals the brute-force nearest-centre
+    rule over all points (the z window only drops centres farther than rad2)."""
+    Nx, Ny, Nz = size
+    dx, dy, dz = (F32(u) for u in units)
+    ps = np.array(points, np.float64).reshape(-1, 3)
+    ps = ps[np.argsort(ps[:, 2], kind="stable")].astype(F32)
+    r2, r1 = F32(rad_outer), F32(rad_inner)
+    x = dx * np.arange(Nx).astype(F32)
+    y = dy * np.arange(Ny).astype(F32)
+    out = np.zeros((Nz, Ny, Nx), F32)
+    Np = ps.shape[0]
+    for k in range(Nz):
+        z = dz * F32(k)
+        ind1 = 0
+        while ps[ind1, 2] < z - r2 and ind1 < Np - 1:
+            ind1 += 1
+        ind2 = Np - 1
+        while ps[ind2, 2] > z + r2 and ind2 > 0:
+            ind2 -= 1
+        dmin = np.full((Ny, Nx), F32(1000000000.) * r2, F32)
+        for n in range(ind1, ind2 + 1):
+            ex = x[None, :] - ps[n, 0]
+            ey = y[:, None] - ps[n, 1]
+            ez = z - ps[n, 2]
+            d = np.sqrt((ex * ex + ey * ey) + ez * ez).astype(F32)
+            dmin = np.minimum(dmin, d)
+        out[k] = np.where(dmin < r2, np.where(dmin < r1, F32(dn_inner), F32(dn_outer)), F32(0))
+    return out

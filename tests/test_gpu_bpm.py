@@ -248,3 +248,24 @@ def test_graph_replay_matches_eager(bpm):
                 (x * 2).sum().item()
         finally:
             _lib.check(lib.bpmh_plan_set_graph(plan, 0))
+
+
+def test_device_sphere_volumes(bpm):
+    from bpm_amd import spheres as S
+    from bpm_amd.workloads import random_spheres, spheres_dn
+    say("case sphere volumes")
+    rng = np.random.default_rng(21)
+    size, units = (96, 80, 64), (.1, .1, .1)
+    pts = rng.uniform(0, 8, (15, 3))
+    got = S.create_dn_buffer(size, units, pts, dn_inner=.04, rad_inner=.6, dn_outer=.02,
+                             rad_outer=1.1).cpu().numpy()
+    ref = O.fill_dn_coated(size, units, pts, dn_inner=.04, rad_inner=.6, dn_outer=.02, rad_outer=1.1)
+    assert np.array_equal(got, ref)
+    c, r, d = random_spheres(rng, size, units, 40, (.3, 1.5), (.01, .05))
+    got = S.spheres_dn_device(size, units, c, r, d).cpu().numpy()
+    assert np.array_equal(got, spheres_dn(size, units, c, r, d))
+    # propagation through the coated-sphere volume (bpm_3d_spheres)
+    u = S.bpm_3d_spheres(size, units, points=pts, dn_inner=.04, rad_inner=.6, dn_outer=.02,
+                         rad_outer=1.1)
+    uref = O.bpm_3d(size, units, .5, None, ref, 1.)
+    assert O.rel_l2(u, uref) < TOL

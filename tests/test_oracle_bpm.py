@@ -286,3 +286,19 @@ def test_sum_intensity_is_z_sum_of_intensity_volume():
     ui, acc = O.bpm_3d((Nx, Ny, Nz), (.1, .1, .1), LAM, None, dn, N0, return_field=False,
                        sum_intensity=True)
     assert np.allclose(acc, ui.astype(np.float64).sum(0), rtol=1e-5)
+
+
+def test_fill_dn_coated_equals_brute_force_nearest_centre():
+    rng = np.random.default_rng(4)
+    size, units = (24, 20, 16), (.1, .12, .15)
+    pts = rng.uniform(0, 2.4, (9, 3))
+    got = O.fill_dn_coated(size, units, pts, dn_inner=.04, rad_inner=.3, dn_outer=.02, rad_outer=.5)
+    Nx, Ny, Nz = size
+    Z, Y, X = np.meshgrid(np.arange(Nz) * units[2], np.arange(Ny) * units[1],
+                          np.arange(Nx) * units[0], indexing="ij")
+    d = np.min([np.sqrt((X - p[0]) ** 2 + (Y - p[1]) ** 2 + (Z - p[2]) ** 2) for p in pts], axis=0)
+    ref = np.where(d < .5, np.where(d < .3, .04, .02), 0.)
+    # voxels within float32 rounding of a sphere surface may differ
+    edge = (np.abs(d - .5) < 1e-5) | (np.abs(d - .3) < 1e-5)
+    assert np.array_equal(got[~edge], ref[~edge].astype(np.float32))
+    assert (got > 0).sum() > 50
