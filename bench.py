@@ -44,6 +44,7 @@ def parse():
                     help="planes of the workload the CPU baseline propagates")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-debye", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the z-loop kernel by kernel instead of replaying a HIP graph")
     return ap.parse_args()
@@ -83,6 +84,26 @@ def debye_extra(torch):
             "note": "octant quadrature + full-grid mirror, outputs on device"}
 
 
+def served_extra(seed):
+    """Mixed-size served stream (grids 128^2..1024^2, depth 64..512, 8-64
+    plane-wave angles per job, Poisson arrivals) through the continuous-
+    batching server, and the same jobs through the plain per-request loop."""
+    from bpm_amd.serving import BpmServer, plain_loop, poisson_jobs
+    jobs = poisson_jobs(seed=seed, n_jobs=24, rate_hz=20.)
+    srv = BpmServer(hbm_budget=48 << 30, slots=4)
+    srv.run(jobs[:2], realtime=False)  # warm-up (plans, propagators)
+    r = srv.run(jobs, realtime=True)
+    p = plain_loop(jobs)
+    return {"workload": "24 jobs, Poisson 20/s, grids {128,256,512,1024}^2, depth U{64..512}, "
+                        "8-64 angles (tilt <= 0.3 rad), spheres as the 256^2 workload; "
+                        "latency = arrival -> results on host",
+            "p50_ms": r["p50_s"] * 1e3, "p99_ms": r["p99_s"] * 1e3,
+            "served_voxel_steps_per_s": r["throughput"],
+            "plain_loop_voxel_steps_per_s": p["throughput"],
+            "served_over_plain": r["throughput"] / p["throughput"],
+            "served_wall_s": r["wall_s"], "plain_wall_s": p["wall_s"]}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -94,18 +115,20 @@ def main():
     import torch
     from bpm_amd import _lib
     from bpm_amd import bpm3d as B
-    from bpm_amd.workloads import plain_large_workload
+    from bpm_amd.spheres import spheres_dn_device
+    from bpm_amd.workloads import gaussian_beam, plain_large_spheres
 
     Nx = Ny = a.nx
     Nz = a.nz
     units, lam, n0 = (.1, .1, .1), .5, 1.33
     t0 = time.perf_counter()
-    u0, dn = plain_large_workload(seed=a.seed, size=(Nx, Ny, Nz), units=units, lam=lam, n0=n0)
-    gen_s = time.perf_counter() - t0
-    print("workload generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
-
+    sph = plain_large_spheres(seed=a.seed, size=(Nx, Ny, Nz), units=units)
+    u0 = gaussian_beam(Nx, Ny, units[0], units[1], lam, n0, 3., -20.)
     dev = torch.device("cuda", 0)
-    dn_g = torch.from_numpy(dn).to(dev)
+    dn_g = spheres_dn_device((Nx, Ny, Nz), units, *sph)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    print("workload generated on device in %.1f s" % gen_s, file=sys.stderr, flush=True)
     u0_g = torch.from_numpy(u0).to(dev).reshape(1, Ny, Nx)
     plane = torch.empty_like(u0_g)
     acc = torch.empty((1, Ny, Nx), dtype=torch.float32, device=dev)
@@ -186,8 +209,13 @@ def main():
     }
     if not a.no_debye:
         out["debye"] = debye_extra(torch)
+    if not a.no_served:
+        out["served"] = served_extra(a.seed)
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(u0, dn, units, lam, n0, min(a.cpu_sample_planes, Nz))
+        from bpm_amd.workloads import spheres_dn
+        planes = min(a.cpu_sample_planes, Nz)
+        dn_s = spheres_dn((Nx, Ny, planes), units, *sph)
+        out["cpu_baseline"] = cpu_baseline(u0, dn_s, units, lam, n0, planes)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 

@@ -11,7 +11,7 @@
 import numpy as np
 
 __all__ = ["random_spheres", "spheres_dn", "gaussian_beam", "plane_wave",
-           "plain_small_workload", "plain_large_workload"]
+           "plain_small_workload", "plain_large_spheres", "plain_large_workload"]
 
 
 def random_spheres(rng, size, units, n, r_range, dn_range):
@@ -83,11 +83,19 @@ def plain_small_workload(seed=0, size=(256, 256, 256), units=(.1, .1, .1)):
     return u0, dn
 
 
-def plain_large_workload(seed=0, size=(1024, 1024, 1024), units=(.1, .1, .1), lam=.5, n0=1.33):
-    """dn and u0 of the 1024^2 x 1024 plain-BPM workload: 400 spheres and a
-    Gaussian beam (w0 = 3 um) focused 20 um into the volume."""
+def plain_large_spheres(seed=0, size=(1024, 1024, 1024), units=(.1, .1, .1)):
+    """Sphere list (centres, radii, dns) of the 1024^2 x 1024 plain-BPM workload."""
     rng = np.random.default_rng(seed)
-    c, r, d = random_spheres(rng, size, units, 400, (1., 4.), (.01, .05))
-    dn = spheres_dn(size, units, c, r, d)
+    return random_spheres(rng, size, units, 400, (1., 4.), (.01, .05))
+
+
+def plain_large_workload(seed=0, size=(1024, 1024, 1024), units=(.1, .1, .1), lam=.5, n0=1.33,
+                         planes=None):
+    """dn and u0 of the 1024^2 x 1024 plain-BPM workload: 400 spheres and a
+    Gaussian beam (w0 = 3 um) focused 20 um into the volume.  ``planes``
+    limits dn to its first planes (same values as the full volume's)."""
+    c, r, d = plain_large_spheres(seed, size, units)
+    nz = size[2] if planes is None else planes
+    dn = spheres_dn((size[0], size[1], nz), units, c, r, d)
     u0 = gaussian_beam(size[0], size[1], units[0], units[1], lam, n0, 3., -20.)
     return u0, dn

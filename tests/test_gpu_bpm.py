@@ -269,3 +269,25 @@ def test_device_sphere_volumes(bpm):
                          rad_outer=1.1)
     uref = O.bpm_3d(size, units, .5, None, ref, 1.)
     assert O.rel_l2(u, uref) < TOL
+
+
+def test_served_results_equal_alone_and_in_mix(bpm):
+    # continuous batching: jobs overlap on the device, yet each job's fields
+    # equal the per-request baseline (each angle alone) bit for bit
+    from bpm_amd.serving import BpmServer, plain_loop, poisson_jobs, job_footprint
+    say("case serving")
+    jobs = poisson_jobs(seed=3, n_jobs=6, rate_hz=200., grids=(64, 128, 256), depth=(16, 48),
+                        angles=(2, 6))
+    budget = 2 * max(job_footprint(j) for j in jobs)
+    srv = BpmServer(hbm_budget=budget, slots=3)
+    r = srv.run(jobs, realtime=True, keep_results=True)
+    p = plain_loop(jobs, keep_results=True)
+    for j in jobs:
+        gu, gi = r["results"][j.jid]
+        pu, pi = p["results"][j.jid]
+        assert np.array_equal(gu, pu) and np.array_equal(gi, pi), j.jid
+    say("   p50 %.1f ms p99 %.1f ms" % (r["p50_s"] * 1e3, r["p99_s"] * 1e3))
+    assert r["p99_s"] >= r["p50_s"] > 0
+    # a job that does not fit the budget is refused up front
+    with pytest.raises(MemoryError):
+        BpmServer(hbm_budget=1000, slots=1).run(jobs[:1])
