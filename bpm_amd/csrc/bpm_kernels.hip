@@ -159,6 +159,14 @@ struct RowArgs {
   int inverse;             // input is F_x u: inverse FFT first
   int screen;              // apply dn[z] screen
   int forward;             // forward FFT_x before storing
+  // adjoint step: v is the gradient G_z (torch convention, 2 dL/du*); the
+  // row pass writes unit_k*Im(conj(G_z) u_z) per item and multiplies by the
+  // conjugate screen
+  int adjoint;
+  const float2* useg;      // forward planes of the segment (batch, nseg, ny, nx)
+  long long useg_bstride;  // elements between items in useg
+  int useg_z;              // plane of z inside the segment
+  float* gitem;            // (batch, ny, nx) per-item gradient of dn[z]
 };
 
 template <int NX>
@@ -205,7 +213,15 @@ k_row(RowArgs a) {
         const float ph = f_mul(-a.unit_k, f_sub(d.x, dn0));
         float s, c;
         sincosf(ph, &s, &c);
-        float2 r = c_mul(v[m], make_float2(c, s));
+        float2 r;
+        if (a.adjoint) {
+          const float2 u = a.useg[(size_t)b * a.useg_bstride +
+                                  ((size_t)a.useg_z * a.ny + y) * NX + x];
+          a.gitem[((size_t)b * a.ny + y) * NX + x] = a.unit_k * (v[m].x * u.y - v[m].y * u.x);
+          r = c_mulc(v[m], make_float2(c, s));
+        } else {
+          r = c_mul(v[m], make_float2(c, s));
+        }
         const int dist = min(min(NX - x - 1, x), disty);
         if (dist < a.absorb) {
           const float arg = f_div(f_mul(BPMH_PI_F, (float)dist), (float)a.absorb);
@@ -261,6 +277,7 @@ struct ColArgs {
   const float* gw;
   const float2* pw;        // plane-wave DC per plane (nz)
   float2* spart;           // scattering partials (batch, nz, ntiles)
+  int conj_h;              // adjoint: multiply by conj(H)
 };
 
 // PHASE 0: FFT_y, H, IFFT_y.  PHASE 1: FFT_y, weighted partials, store.
@@ -320,7 +337,7 @@ k_col(ColArgs a) {
       const int qy = min(ky, NY - ky);
       const float2 h = (a.hmode == 0) ? a.H[(size_t)qy * a.nx + qx]
                                       : prop_f32(kx2, kcomp2_f32(ky, NY, a.dy), an, a.dz);
-      v[m] = c_mul(v[m], h);
+      v[m] = a.conj_h ? c_mulc(v[m], h) : c_mul(v[m], h);
     }
     if (a.sw) {
       // scattering reductions of bpm_3d.py:243-247,335-341 on the k-space
@@ -832,6 +849,84 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, 
     HIP_TRY(hipGraphLaunch(plan->gexec, (hipStream_t)stream));
     return BPMH_OK;
   }
+  HIP_TRY(hipGetLastError());
+  return BPMH_OK;
+}
+
+// ----------------------------------------------------------------------------
+// adjoint (gradient of a loss on the last plane with respect to real dn)
+// ----------------------------------------------------------------------------
+// grad[i] += sum_b gitem[b][i] in item order (deterministic)
+__global__ void k_grad_reduce(const float* __restrict__ gitem, float* __restrict__ grad,
+                              int batch, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = grad[i];
+  for (int b = 0; b < batch; ++b) s += gitem[(size_t)b * n + i];
+  grad[i] = s;
+}
+
+extern "C" int bpmh_adjoint_segment(bpmh_plan plan, int batch, int z_lo, int z_hi, float* g_,
+                                    const float* u_seg_, const float* dn, const float* H_,
+                                    float unit_k, int absorb, float* grad_dn, float* gitem,
+                                    void* stream) {
+  float2* g = (float2*)g_;
+  const float2* u_seg = (const float2*)u_seg_;
+  const float2* H = (const float2*)H_;
+  if (!plan || !g || !u_seg || !dn || !H || !grad_dn || !gitem)
+    return bpmh_set_error(BPMH_ERR_INVALID, "null argument");
+  if (batch < 1 || z_lo < 0 || z_hi < z_lo) return bpmh_set_error(BPMH_ERR_INVALID, "bad range");
+  if (absorb < 0) return bpmh_set_error(BPMH_ERR_INVALID, "absorbing width must be >= 0");
+  if (z_hi == z_lo) return BPMH_OK;
+  const int nx = plan->nx, ny = plan->ny;
+  const int nseg = z_hi - z_lo + 1;
+  hipStream_t s = (hipStream_t)stream;
+  const row_fn row = pick_row(nx);
+  const col_fn col0 = pick_col<0>(ny);
+
+  RowArgs ra;
+  memset(&ra, 0, sizeof ra);
+  ra.plane = g;
+  ra.tw = plan->tw_x;
+  ra.dn = dn;
+  ra.dn_kind = BPMH_DN_F32;
+  ra.ny = ny;
+  ra.nz = z_hi + 1;
+  ra.unit_k = unit_k;
+  ra.inv_n = 1.0f / ((float)nx * (float)ny);
+  ra.absorb = absorb;
+  ra.adjoint = 1;
+  ra.useg = u_seg;
+  ra.useg_bstride = (long long)nseg * ny * nx;
+  ra.gitem = gitem;
+
+  ColArgs ca;
+  memset(&ca, 0, sizeof ca);
+  ca.plane = g;
+  ca.tw = plan->tw_y;
+  ca.nx = nx;
+  ca.hmode = 0;
+  ca.H = H;
+  ca.conj_h = 1;
+
+  const size_t n = (size_t)nx * ny;
+  for (int z = z_hi; z > z_lo; --z) {
+    ra.z = z;
+    ra.useg_z = z - z_lo;
+    ra.inverse = z != z_hi;
+    ra.screen = 1;
+    ra.forward = 1;
+    row(ra, batch, s);
+    hipLaunchKernelGGL(k_grad_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, gitem,
+                       grad_dn + (size_t)z * n, batch, n);
+    col0(ca, batch, s);
+  }
+  // back to real space: the gradient with respect to u[z_lo]
+  ra.adjoint = 0;
+  ra.screen = 0;
+  ra.inverse = 1;
+  ra.forward = 0;
+  row(ra, batch, s);
   HIP_TRY(hipGetLastError());
   return BPMH_OK;
 }

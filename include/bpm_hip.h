@@ -129,6 +129,20 @@ int bpmh_fill_dn_coated(int nx, int ny, int nz, float dx, float dy, float dz,
 int bpmh_spheres_dn(int nx, int ny, int nz, double dx, double dy, double dz,
                     const double* spheres, int n_spheres, float* dn, void* stream);
 
+/* Adjoint split-step over planes z_lo..z_hi of a static-propagator
+ * (mean_mode ignore / Bpm3d) run with real float32 dn shared by `batch`
+ * fields.  g: (batch, ny, nx) c64, on entry G = 2 dL/du*[z_hi] (the
+ * gradient convention of torch), on exit G for u[z_lo].  u_seg: the forward
+ * planes u[z_lo..z_hi] (batch, z_hi-z_lo+1, ny, nx) c64 (recomputed from a
+ * checkpoint by bpmh_propagate with out_field).  dn: the full (nz, ny, nx)
+ * volume; grad_dn (nz, ny, nx) f32 receives += dL/ddn for planes
+ * z_lo+1..z_hi; gitem: (batch, ny, nx) f32 scratch.  H: the forward table.
+ * No reference counterpart (the reference has no gradient; its
+ * bpm/bpm/bpm_3d_inverse.py is a non-differentiable phase-ratio experiment). */
+int bpmh_adjoint_segment(bpmh_plan plan, int batch, int z_lo, int z_hi, float* g,
+                         const float* u_seg, const float* dn, const float* H, float unit_k,
+                         int absorb, float* grad_dn, float* gitem, void* stream);
+
 /* Debye-Wolf kernel variants */
 #define BPMH_DW_INDEX 0  /* debye_wolf, bpm/psf/kernels/psf_debye.cl:24-113 */
 #define BPMH_DW_SIN 1    /* debye_wolf_plane / debye_wolf_at, psf_debye.cl:117-269 */

@@ -1,0 +1,141 @@
+"""GPU checks of the hand-written adjoint (bpm_amd.adjoint) against a float64
+torch autograd reference of the same split-step map and finite differences."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LAM, N0 = .5, 1.33
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def reference_f64(dn, u0, units, lam, n0, absorbing_width=0):
+    """u_z = exp(-i k0 dz dn_z) T * IFFT2(H FFT2(u_{z-1})) in complex128 (torch autograd)."""
+    import torch as t
+    nz, ny, nx = dn.shape
+    dx, dy, dz = units
+    k0 = 2. * np.pi / lam
+    kx = 2. * np.pi * np.fft.fftfreq(nx, dx)
+    ky = 2. * np.pi * np.fft.fftfreq(ny, dy)
+    tmp = n0 ** 2 * k0 ** 2 - kx[None, :] ** 2 - ky[:, None] ** 2
+    H = t.from_numpy(np.where(tmp >= 0, np.exp(-1j * dz * np.sqrt(np.maximum(tmp, 0))), 0)).to(dn.device)
+    T = None
+    if absorbing_width > 0:
+        i, j = np.arange(nx), np.arange(ny)
+        d = np.minimum(np.minimum(nx - 1 - i, i)[None, :], np.minimum(ny - 1 - j, j)[:, None])
+        T = t.from_numpy(np.where(d < absorbing_width,
+                                  .5 * (1 - np.cos(np.pi * d / absorbing_width)), 1.)).to(dn.device)
+    u = u0.to(t.complex128)
+    for z in range(1, nz):
+        u = t.fft.ifft2(t.fft.fft2(u) * H)
+        u = u * t.exp(-1j * k0 * dz * dn[z])
+        if T is not None:
+            u = u * T
+    return u
+
+
+def setup(nz, ny, nx, Bn, seed=0, units=(.1, .1, .1)):
+    import torch as t
+    from bpm_amd.spheres import spheres_dn_device
+    from bpm_amd.workloads import random_spheres
+    rng = np.random.default_rng(seed)
+    size = (nx, ny, nz)
+    dn = spheres_dn_device(size, units, *random_spheres(rng, size, units, 30, (.4, 1.2), (.01, .04)))
+    dn2 = spheres_dn_device(size, units, *random_spheres(rng, size, units, 30, (.4, 1.2), (.01, .04)))
+    th = rng.uniform(-.2, .2, (Bn, 2))
+    x = np.arange(nx) * units[0]
+    y = np.arange(ny) * units[1]
+    k = 2 * np.pi * N0 / LAM
+    u0 = np.exp(-1j * k * (np.sin(th[:, 0])[:, None, None] * x[None, None, :] +
+                           np.sin(th[:, 1])[:, None, None] * y[None, :, None])).astype(np.complex64)
+    return dn, dn2, t.from_numpy(u0).cuda()
+
+
+@pytest.mark.parametrize("absorb", [0, 5])
+def test_adjoint_vs_float64_autograd(absorb):
+    import torch as t
+    from bpm_amd import adjoint
+    nz, ny, nx, Bn = 24, 48, 64, 3
+    say("case adjoint vs f64 autograd absorb=%d" % absorb)
+    dn, dn2, u0 = setup(nz, ny, nx, Bn, seed=1)
+    target = adjoint.propagate(dn2, u0, lam=LAM, n0=N0, absorbing_width=absorb).detach()
+    dn_a = dn.clone().requires_grad_(True)
+    u0_a = u0.clone().requires_grad_(True)
+    out = adjoint.propagate(dn_a, u0_a, lam=LAM, n0=N0, absorbing_width=absorb, checkpoint=5)
+    loss = (out - target).abs().pow(2).sum()
+    loss.backward()
+    dn_r = dn.double().clone().requires_grad_(True)
+    u0_r = u0.to(t.complex128).clone().requires_grad_(True)
+    ref = reference_f64(dn_r, u0_r, (.1, .1, .1), LAM, N0, absorb)
+    loss_r = (ref - target.to(t.complex128)).abs().pow(2).sum()
+    loss_r.backward()
+    g, gr = dn_a.grad.double(), dn_r.grad
+    e = ((g - gr).norm() / gr.norm()).item()
+    eu = ((u0_a.grad.to(t.complex128) - u0_r.grad).norm() / u0_r.grad.norm()).item()
+    say("   loss %.6e vs f64 %.6e; grad_dn rel-L2 %.2e; grad_u0 rel-L2 %.2e"
+        % (loss.item(), loss_r.item(), e, eu))
+    assert abs(loss.item() - loss_r.item()) / loss_r.item() < 1e-4
+    assert e < 1e-4 and eu < 1e-4
+    assert g[0].abs().max().item() == 0  # dn[0] does not enter the propagation
+
+
+def test_checkpoint_interval_does_not_change_gradient():
+    from bpm_amd import adjoint
+    nz, ny, nx, Bn = 30, 32, 32, 2
+    dn, dn2, u0 = setup(nz, ny, nx, Bn, seed=2)
+    grads = []
+    for K in (1, 7, 29, 64):
+        d = dn.clone().requires_grad_(True)
+        out = adjoint.propagate(d, u0, lam=LAM, n0=N0, checkpoint=K)
+        (out - u0).abs().pow(2).sum().backward()
+        grads.append(d.grad.clone())
+    for g in grads[1:]:
+        assert (g == grads[0]).all()
+
+
+def test_adjoint_full_size_vs_finite_differences():
+    # the differentiable workload: 512^2 x 256 slices, 16 angles, loss = L2
+    # between exit fields and targets from a second seeded dn, checkpoint 16;
+    # float64 central differences of the float64 reference on 32 voxels
+    import torch as t
+    from bpm_amd import adjoint
+    nz, ny, nx, Bn = 256, 512, 512, 16
+    say("case adjoint 512^2 x 256 x 16")
+    dn, dn2, u0 = setup(nz, ny, nx, Bn, seed=3)
+    target = adjoint.propagate(dn2, u0, lam=LAM, n0=N0).detach().to(t.complex128)
+    d = dn.clone().requires_grad_(True)
+    out = adjoint.propagate(d, u0, lam=LAM, n0=N0, checkpoint=16)
+    ((out.to(t.complex128) - target).abs().pow(2).sum()).backward()
+    g = d.grad.double()
+    rng = np.random.default_rng(5)
+    # voxels inside spheres and outside, planes 1..nz-1
+    nzv = t.nonzero(dn[1:] > 0)
+    pick = nzv[t.from_numpy(rng.choice(nzv.shape[0], 16, replace=False)).cuda()]
+    pick[:, 0] += 1
+    rand = t.from_numpy(np.stack([rng.integers(1, nz, 16), rng.integers(0, ny, 16),
+                                  rng.integers(0, nx, 16)], 1)).cuda()
+    vox = t.cat([pick, rand])
+    base = dn.double()
+    eps = 1e-4
+    errs = []
+    with t.no_grad():
+        for (z, y, x) in vox.tolist():
+            vals = []
+            for sgn in (1, -1):
+                p = base.clone()
+                p[z, y, x] += sgn * eps
+                r = reference_f64(p, u0, (.1, .1, .1), LAM, N0)
+                vals.append((r - target).abs().pow(2).sum().item())
+            fd = (vals[0] - vals[1]) / (2 * eps)
+            errs.append((g[z, y, x].item(), fd))
+    a = np.array(errs)
+    scale = np.abs(a[:, 1]).max()
+    err = np.abs(a[:, 0] - a[:, 1]).max() / scale
+    say("   max |adjoint - FD| / max|FD| over 32 voxels: %.2e" % err)
+    assert err < 1e-3
