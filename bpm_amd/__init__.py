@@ -10,10 +10,13 @@ as in the reference; torch GPU tensors are accepted as inputs.
 """
 from .bpm3d import bpm_3d
 from .bpmclass import Bpm3d
-from .focus_fields import (psf, psf_u0, psf_lightsheet, focus_field_debye, focus_field_debye_plane,
-                  focus_field_debye_at, focus_field_debye_gauss, focus_field_debye3)
+from .focus_fields import (psf, psf_u0, psf_lightsheet, psf_cylindrical, psf_cylindrical_u0,
+                           focus_field_debye, focus_field_debye_plane, focus_field_debye_at,
+                           focus_field_debye_gauss, focus_field_debye3, focus_field_cylindrical,
+                           focus_field_cylindrical_plane)
 
-__all__ = ["bpm_3d", "Bpm3d", "psf", "psf_u0", "psf_lightsheet", "focus_field_debye",
-           "focus_field_debye_plane", "focus_field_debye_at", "focus_field_debye_gauss",
-           "focus_field_debye3"]
+__all__ = ["bpm_3d", "Bpm3d", "psf", "psf_u0", "psf_lightsheet", "psf_cylindrical",
+           "psf_cylindrical_u0", "focus_field_debye", "focus_field_debye_plane",
+           "focus_field_debye_at", "focus_field_debye_gauss", "focus_field_debye3",
+           "focus_field_cylindrical", "focus_field_cylindrical_plane"]
 __version__ = "0.1.0"

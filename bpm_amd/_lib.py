@@ -44,6 +44,7 @@ SIGNATURES = {
                                     _P, _P, _P, _P]),
     "bpmh_debye_mirror": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "bpmh_bessel_fill": (_I, [_I, _P, _P, _I, _P]),
+    "bpmh_cylindrical": (_I, [_I, _I, _F, _F, _F, _F, _F, _F, _I, _I, _P, _P, _P, _P, _P]),
     "bpmh_fill_dn_coated": (_I, [_I, _I, _I, _F, _F, _F, _P, _I, _F, _F, _F, _F, _P, _P]),
     "bpmh_spheres_dn": (_I, [_I, _I, _I, _D, _D, _D, _P, _I, _P, _P]),
     "bpmh_bessel_host": (_I, [_I, _P, _P, _I]),

@@ -318,3 +318,57 @@ extern "C" int bpmh_bessel_host(int n, const double* x, double* out, int count) 
   for (int i = 0; i < count; ++i) out[i] = bpmbessel::jn(n, x[i]);
   return BPMH_OK;
 }
+
+// ----------------------------------------------------------------------------
+// cylindrical lens (bpm/psf/kernels/psf_cylindrical.cl:74-185): per (y, z),
+//   E = sum_t w(t) sqrt(cos t) e^{i(ky sin t + kz cos t)} (1, cos t, sin t)
+// over S+1 samples t = -alpha + i_t dt, dt = 2 alpha/(S-1); end weights by
+// index (psf_cylindrical) or by float equality with -+alpha (_plane).
+// ----------------------------------------------------------------------------
+__global__ void k_cylindrical(int ny, int nz, float y1, float y2, float z1, float z2, float lam,
+                              float alpha, int S, int plane_variant, float2* ex, float2* ey,
+                              float2* ez, float* I) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= ny) return;
+  const float y = axis_coord(i, ny, y1, y2);
+  const float z = axis_coord(j, nz, z1, z2);
+  const float kf = f_div(f_mul(2.f, 3.141592653589793f), lam);
+  const float ky = f_mul(kf, y), kz = f_mul(kf, z);
+  const float dt = f_div(f_mul(2.f, alpha), (float)S - 1.f);
+  float2 sx = make_float2(0.f, 0.f), sy = sx, sz = sx;
+  for (int it = 0; it <= S; ++it) {
+    const float t = f_add(-alpha, f_mul((float)it, dt));
+    double sd, cd;
+    sincos((double)t, &sd, &cd);
+    const float co = (float)cd, si = (float)sd;
+    float w;
+    if (plane_variant) w = (t == -alpha || t == alpha) ? .5f : 1.f;
+    else w = (it == 0 || it == S) ? .5f : 1.f;
+    w = f_mul(w, f_mul(dt, __fsqrt_rn(co)));
+    float sn, cs;
+    sincosf(f_add(f_mul(ky, si), f_mul(kz, co)), &sn, &cs);
+    sx.x += w * cs; sx.y += w * sn;
+    const float wc = f_mul(w, co), ws = f_mul(w, si);
+    sy.x += wc * cs; sy.y += wc * sn;
+    sz.x += ws * cs; sz.y += ws * sn;
+  }
+  const size_t o = (size_t)j * ny + i;
+  if (ex) ex[o] = sx;
+  if (ey) ey[o] = sy;
+  if (ez) ez[o] = sz;
+  if (I) I[o] = sx.x * sx.x + sx.y * sx.y + sy.x * sy.x + sy.y * sy.y + sz.x * sz.x + sz.y * sz.y;
+}
+
+extern "C" int bpmh_cylindrical(int ny, int nz, float y1, float y2, float z1, float z2, float lam,
+                                float alpha, int int_steps, int plane_variant, float* ex,
+                                float* ey, float* ez, float* I, void* stream) {
+  if (ny < 1 || nz < 1 || nz > 65535) return bpmh_set_error(BPMH_ERR_INVALID, "bad grid");
+  if (int_steps < 2) return bpmh_set_error(BPMH_ERR_INVALID, "int_steps must be >= 2");
+  hipLaunchKernelGGL(k_cylindrical, dim3((ny + 63) / 64, nz), dim3(64), 0, (hipStream_t)stream, ny,
+                     nz, y1, y2, z1, z2, lam, alpha, int_steps, plane_variant, (float2*)ex,
+                     (float2*)ey, (float2*)ez, I);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(e));
+  return BPMH_OK;
+}

@@ -13,7 +13,8 @@ from . import _lib
 
 __all__ = ["psf", "psf_u0", "psf_lightsheet", "focus_field_debye", "focus_field_debye_plane",
            "focus_field_debye_at", "focus_field_debye_gauss", "focus_field_debye3",
-           "focus_field_debye_device", "bessel_jn"]
+           "focus_field_debye_device", "bessel_jn", "psf_cylindrical", "psf_cylindrical_u0",
+           "focus_field_cylindrical", "focus_field_cylindrical_plane"]
 
 
 def _alphas(NA, n0):
@@ -209,3 +210,69 @@ def bessel_jn(n, x):
     _lib.call("bpmh_bessel_fill", int(n), _lib.ptr(xg), _lib.ptr(out), int(xg.numel()),
               _lib.stream_handle())
     return _lib.to_numpy(out).reshape(np.shape(x))
+
+
+def _cyl_call(Ny, Nz, y1, y2, z1, z2, lam, alpha, steps, plane_variant, outputs):
+    ex = _lib.empty((Nz, Ny), np.complex64) if outputs[0] else None
+    ey = _lib.empty((Nz, Ny), np.complex64) if outputs[1] else None
+    ez = _lib.empty((Nz, Ny), np.complex64) if outputs[2] else None
+    u = _lib.empty((Nz, Ny), np.float32) if outputs[3] else None
+    f = lambda v: float(np.float32(v))
+    _lib.call("bpmh_cylindrical", int(Ny), int(Nz), f(y1), f(y2), f(z1), f(z2), f(lam), f(alpha),
+              int(steps), int(plane_variant), _lib.ptr(ex), _lib.ptr(ey), _lib.ptr(ez),
+              _lib.ptr(u), _lib.stream_handle())
+    return u, ex, ey, ez
+
+
+def focus_field_cylindrical(shape, units, lam=.5, NA=.3, n0=1., n_integration_steps=100):
+    """Focus field of a cylindrical lens (bpm/psf/_focus_fields_cylindrical.py:32-79;
+    C. J. R. Sheppard, Appl. Opt. 52, 538 (2013)): the (y, z) field repeated
+    along x; returns u, ex, ey, ez of shape (Nz, Ny, Nx)."""
+    Nx, Ny, Nz = shape
+    dx, dy, dz = units
+    alpha = np.arcsin(NA / n0)
+    res = _cyl_call(Ny, Nz, -dy * (Ny - 1) / 2., dy * (Ny - 1) / 2., -dz * (Nz - 1) / 2.,
+                    dz * (Nz - 1) / 2., lam / n0, alpha, n_integration_steps, 0,
+                    (True, True, True, True))
+    return tuple(np.repeat(_lib.to_numpy(v)[..., np.newaxis], Nx, axis=-1) for v in res)
+
+
+def focus_field_cylindrical_plane(shape=(128, 128), units=(.1, .1), z=0., lam=.5, NA=.6, n0=1.,
+                                  ex_g=None, n_integration_steps=200):
+    """Ex of a cylindrical lens on the plane z (_focus_fields_cylindrical.py:83-134),
+    (Ny, Nx); fills the GPU tensor ex_g if given, else returns numpy."""
+    Nx, Ny = shape
+    dx, dy = units
+    alpha = np.arcsin(NA / n0)
+    if ex_g is not None and tuple(ex_g.shape) != (Ny, Nx):
+        raise AssertionError("ex_g must have shape (Ny,Nx)")
+    _, ex, _, _ = _cyl_call(Ny, 1, -dy * (Ny - 1) / 2., dy * (Ny - 1) / 2., z, z, lam / n0, alpha,
+                            n_integration_steps, 1, (True, False, False, False))
+    full = ex.reshape(Ny, 1).expand(Ny, Nx)
+    if ex_g is not None:
+        ex_g.copy_(full)
+        return None
+    return _lib.to_numpy(full.contiguous())
+
+
+def psf_cylindrical(shape, units, lam, NA, n0=1., return_field=False, n_integration_steps=100):
+    """PSF of a cylindrical lens (bpm/psf/psf_functions.py:122-137): u, or
+    (u, ex) if return_field.  (The reference unpacks the four return values of
+    focus_field_cylindrical into two names and raises; this returns what its
+    docstring and psf_cylindrical_u0 expect.)"""
+    u, ex, _, _ = focus_field_cylindrical(shape=shape, units=units, lam=lam, NA=NA, n0=n0,
+                                          n_integration_steps=n_integration_steps)
+    if return_field:
+        return u, ex
+    return u
+
+
+def psf_cylindrical_u0(shape, units, zfoc, lam, NA, n0=1., n_integration_steps=200):
+    """Initial plane of a cylindrical-lens beam focused at zfoc
+    (psf_functions.py:140-156): conj(Ex) of psf_cylindrical((Nx,Ny,4),
+    (dx,dy,2 zfoc/3)) at its first plane."""
+    Nx, Ny = shape
+    dx, dy = units
+    u, ex = psf_cylindrical(shape=(Nx, Ny, 4), units=(dx, dy, 2. * zfoc / 3.), lam=lam, NA=NA,
+                            n0=n0, n_integration_steps=n_integration_steps, return_field=True)
+    return ex[0, ...].conjugate()

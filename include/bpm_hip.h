@@ -176,6 +176,17 @@ int bpmh_debye_mirror(int nx0, int ny0, int nz0, const float* u_oct, const float
                       const float* ey_oct, const float* ez_oct, float* u, float* ex,
                       float* ey, float* ez, void* stream);
 
+/* Cylindrical-lens focus field on an (nz, ny) grid (y = y1 + i (y2-y1)/(ny-1),
+ * same for z; nz = 1 means z = z1): Ex = sum_t w sqrt(cos t) e^{i(ky sin t +
+ * kz cos t)}, Ey, Ez the same times cos t, sin t, I = |Ex|^2+|Ey|^2+|Ez|^2,
+ * S+1 samples over [-alpha, alpha + dt].  plane_variant 0 replaces the
+ * psf_cylindrical launch (bpm/psf/_focus_fields_cylindrical.py:61-68), 1 the
+ * psf_cylindrical_plane launch (:124-130) whose end weights test t == -+alpha.
+ * Any output may be NULL. */
+int bpmh_cylindrical(int ny, int nz, float y1, float y2, float z1, float z2, float lam,
+                     float alpha, int int_steps, int plane_variant, float* ex, float* ey,
+                     float* ez, float* I, void* stream);
+
 /* out[i] = J_n(x[i]) for n in {0,1,2} with this library's Bessel functions
  * (device arrays).  Replaces the bessel_fill self-test kernel,
  * bpm/psf/kernels/bessel.cl:120-129 (bpm/psf/_focus_fields_debye.py:34-42). */

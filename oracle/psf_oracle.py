@@ -31,6 +31,8 @@ mirror_octant     focus_field_debye host mirror  _focus_fields_debye.py:116-143
                   the TIFF profile fixtures of the reference
                   (tests/data/psf_NA_0.8_dx_0.1_n_*.tif, central lines).
 focus_field_debye _focus_fields_debye.py:48-146 -- composition of the above.
+cylindrical       psf_cylindrical[_plane] kernels psf_cylindrical.cl:74-185
+                  pinned: trapezoid on [-alpha, alpha+dt] vs scipy quad.
 """
 import numpy as np
 from scipy.special import jv
@@ -43,7 +45,7 @@ PI_D = 3.141592653589793
 __all__ = ["debye_wolf_grid", "debye_wolf_points", "mirror_octant",
            "focus_field_debye", "focus_field_debye_plane",
            "focus_field_debye_at", "focus_field_debye_gauss",
-           "focus_field_debye3", "bessel_jn"]
+           "focus_field_debye3", "bessel_jn", "cylindrical"]
 
 
 def bessel_jn(n, x):
@@ -320,3 +322,43 @@ def _mirror_half_offset(u, ex, ey, ez, shape, conj_high_z):
                 v = np.conjugate(v)
             o[sl] = v
     return tuple(outs)
+
+
+def cylindrical(Ny, Nz, y1, y2, z1, z2, lam, alpha, int_steps, plane_variant=False,
+                precision="f32"):
+    """psf_cylindrical / psf_cylindrical_plane kernels (bpm/psf/kernels/psf_cylindrical.cl:74-185)
+    on an (Nz, Ny) grid; returns ex, ey, ez, I.  M_PI is the float literal of
+    that file (:16).  Pinned (tests/test_oracle_psf.py): trapezoid rule on
+    [-alpha, alpha + dt] against scipy.integrate.quad of the same integrand."""
+    f64 = precision == "f64"
+    rdt = np.float64 if f64 else F32
+    S = int(int_steps)
+    y = _axis(Ny, y1, y2, f64)
+    z = _axis(Nz, z1, z2, f64)
+    if f64:
+        kf = 2. * np.pi / lam
+    else:
+        kf = (F32(2.) * F32(3.141592653589793)) / F32(lam)
+    ky = (kf * y).astype(rdt)[None, :]
+    kz = (kf * z).astype(rdt)[:, None]
+    alpha = rdt(alpha)
+    dt = (rdt(2.) * alpha) / rdt(S - 1.)
+    acc = [np.zeros((Nz, Ny), rdt) for _ in range(6)]
+    for it in range(S + 1):
+        t = -alpha + rdt(it) * dt
+        co, si = _c(t, f64), _s(t, f64)
+        if plane_variant:
+            w = rdt(.5) if (t == -alpha or t == alpha) else rdt(1.)
+        else:
+            w = rdt(.5) if (it == 0 or it == S) else rdt(1.)
+        sq = np.sqrt(co) if f64 else F32(np.sqrt(co))
+        w = w * (dt * sq)
+        arg = ky * si + kz * co
+        c, s = _c(arg, f64), _s(arg, f64)
+        for k, ww in enumerate((w, w * co, w * si)):
+            acc[2 * k] += ww * c
+            acc[2 * k + 1] += ww * s
+    cdt = np.complex128 if f64 else np.complex64
+    ex, ey, ez = ((acc[2 * k] + 1j * acc[2 * k + 1]).astype(cdt) for k in range(3))
+    I = (acc[0] ** 2 + acc[1] ** 2 + acc[2] ** 2 + acc[3] ** 2 + acc[4] ** 2 + acc[5] ** 2).astype(rdt)
+    return ex, ey, ez, I

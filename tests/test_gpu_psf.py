@@ -116,3 +116,24 @@ def test_full_size_debye_planes(bpm):
             e = O.rel_l2(g, r)
             say("   plane %d rel-L2 %.2e" % (k, e))
             assert e < TOL
+
+
+def test_cylindrical_lens(bpm):
+    say("case cylindrical")
+    shape, units = (16, 40, 24), (.1, .1, .1)
+    got = bpm.focus_field_cylindrical(shape, units, lam=.5, NA=.5, n0=1.2, n_integration_steps=90)
+    ref = P.cylindrical(40, 24, -.1 * 39 / 2., .1 * 39 / 2., -.1 * 23 / 2., .1 * 23 / 2., .5 / 1.2,
+                        np.arcsin(.5 / 1.2), 90)
+    for g, r in zip(got, (ref[3], ref[0], ref[1], ref[2])):
+        assert g.shape == (24, 40, 16)
+        assert O.rel_l2(g, np.repeat(r[..., None], 16, -1)) < TOL
+    u, ex = bpm.psf_cylindrical(shape, units, .5, .5, n0=1.2, return_field=True, n_integration_steps=90)
+    assert O.rel_l2(u, got[0]) < 1e-7
+    u0 = bpm.psf_cylindrical_u0((16, 40), (.1, .1), 2., .5, .5, n0=1.2, n_integration_steps=90)
+    r0 = P.cylindrical(40, 4, -.1 * 39 / 2., .1 * 39 / 2., -2., 2., .5 / 1.2, np.arcsin(.5 / 1.2), 90)[0]
+    assert O.rel_l2(u0, np.repeat(np.conj(r0[0])[:, None], 16, -1)) < TOL
+    pl = bpm.focus_field_cylindrical_plane((16, 40), (.1, .1), z=-.7, lam=.5, NA=.5, n0=1.2,
+                                           n_integration_steps=90)
+    rp = P.cylindrical(40, 1, -.1 * 39 / 2., .1 * 39 / 2., -.7, -.7, .5 / 1.2, np.arcsin(.5 / 1.2), 90,
+                       plane_variant=True)[0]
+    assert O.rel_l2(pl, np.repeat(rp[0][:, None], 16, -1)) < TOL

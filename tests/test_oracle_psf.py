@@ -185,3 +185,22 @@ def test_bessel_oracle_known_values():
     assert abs(P.bessel_jn(0, 1.0) - 0.7651976865579666) < 1e-15
     x = np.linspace(0.1, 50, 200)
     assert np.allclose(P.bessel_jn(0, x) + P.bessel_jn(2, x), 2 / x * P.bessel_jn(1, x), atol=1e-13)
+
+
+def test_cylindrical_is_trapezoid_on_extended_interval():
+    lam, alpha, S = .5, np.arcsin(.6), 800
+    dt = 2 * alpha / (S - 1)
+    ex, ey, ez, I = P.cylindrical(5, 4, -.6, .6, -1., .8, lam, alpha, S, precision="f64")
+    y = np.linspace(-.6, .6, 5)
+    z = np.linspace(-1., .8, 4)
+    k = 2 * np.pi / lam
+    for (j, zz) in enumerate(z):
+        for (i, yy) in enumerate(y):
+            for m, f in enumerate((lambda t: 1., np.cos, np.sin)):
+                re = quad(lambda t: np.sqrt(np.cos(t)) * f(t) * np.cos(k * (yy * np.sin(t) + zz * np.cos(t))),
+                          -alpha, alpha + dt, limit=200)[0]
+                im = quad(lambda t: np.sqrt(np.cos(t)) * f(t) * np.sin(k * (yy * np.sin(t) + zz * np.cos(t))),
+                          -alpha, alpha + dt, limit=200)[0]
+                got = (ex, ey, ez)[m][j, i]
+                assert abs(got - (re + 1j * im)) < 2e-5
+    assert np.allclose(I, np.abs(ex) ** 2 + np.abs(ey) ** 2 + np.abs(ez) ** 2)
