@@ -27,6 +27,10 @@ def get_plan(Nx, Ny):
     if p is None:
         h = ctypes.c_void_p()
         _lib.call("bpmh_plan_create", int(Nx), int(Ny), ctypes.byref(h))
+        from .tuning import lookup
+        v = lookup(int(Nx), int(Ny), 1)
+        if v is not None:
+            _lib.call("bpmh_plan_set_variant", h, int(v[0]), int(v[1]))
         p = _plans[key] = h
     return p
 

@@ -50,17 +50,40 @@ extern "C" const char* bpmh_last_error(void) { return g_err.c_str(); }
 extern "C" int bpmh_version(void) { return BPMH_VERSION; }
 
 // ----------------------------------------------------------------------------
-// per-size launch configuration
+// per-size launch configurations: variant V in {0,1,2} scales the default
+// number of lines (rows) or columns per work-group by 1/2, 1, 2, clamped to
+// 1..16 lines, 1024 threads and 128 KiB of LDS (gfx950 has 160 KiB per CU);
+// the autotuner picks V per size.
 // ----------------------------------------------------------------------------
+constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+constexpr int pow2floor(int v) { return v < 2 ? 1 : 2 * pow2floor(v / 2); }
+constexpr int scale_v(int base, int V) { return V == 0 ? base / 2 : (V == 1 ? base : base * 2); }
+
 template <int N>
-struct Cfg {
+struct Base {
   static constexpr int E = (N <= 32) ? 4 : 8;
   static constexpr int T = N / E;
-  static constexpr int LINES = (256 / T) > 16 ? 16 : ((256 / T) < 1 ? 1 : 256 / T);
-  static constexpr int COLS0 = 512 / T;
-  static constexpr int COLS = N >= 4096 ? 1 : (COLS0 > 16 ? 16 : (COLS0 < 2 ? 2 : COLS0));
-  static constexpr int ROW_THREADS = LINES * T;
-  static constexpr int COL_THREADS = COLS * T;
+  static constexpr int LDS_LINE = (N + N / 16) * 8;  // bytes per padded line
+};
+
+template <int N, int V>
+struct RowCfg : Base<N> {
+  using Base<N>::T;
+  static constexpr int L0 = clampi(256 / T, 1, 16);
+  static constexpr int LINES =
+      clampi(clampi(scale_v(L0, V), 1, 16), 1, (1024 / T) < 1 ? 1 : 1024 / T);
+  static constexpr int THREADS = LINES * T;
+};
+
+template <int N, int V>
+struct ColCfg : Base<N> {
+  using Base<N>::T;
+  using Base<N>::LDS_LINE;
+  static constexpr int C0 = N >= 4096 ? 1 : clampi(512 / T, 2, 16);
+  static constexpr int CMAX0 = 131072 / LDS_LINE < 1 ? 1 : 131072 / LDS_LINE;
+  static constexpr int CMAX = pow2floor(clampi(CMAX0 < 1024 / T ? CMAX0 : 1024 / T, 1, 16));
+  static constexpr int COLS = clampi(scale_v(C0, V), 1, CMAX);
+  static constexpr int THREADS = COLS * T;
 };
 
 // ----------------------------------------------------------------------------
@@ -169,10 +192,10 @@ struct RowArgs {
   float* gitem;            // (batch, ny, nx) per-item gradient of dn[z]
 };
 
-template <int NX>
-__global__ void __launch_bounds__(Cfg<NX>::ROW_THREADS)
+template <int NX, int V>
+__global__ void __launch_bounds__((RowCfg<NX, V>::THREADS))
 k_row(RowArgs a) {
-  constexpr int E = Cfg<NX>::E, T = Cfg<NX>::T, LINES = Cfg<NX>::LINES;
+  constexpr int E = RowCfg<NX, V>::E, T = RowCfg<NX, V>::T, LINES = RowCfg<NX, V>::LINES;
   __shared__ float2 lds[LINES][lds_len<NX>()];
   const int line = threadIdx.x / T;
   const int t = threadIdx.x - line * T;
@@ -282,11 +305,11 @@ struct ColArgs {
 
 // PHASE 0: FFT_y, H, IFFT_y.  PHASE 1: FFT_y, weighted partials, store.
 // PHASE 2: weighted dn0, H(n0+dn0), IFFT_y.
-template <int NY, int PHASE>
-__global__ void __launch_bounds__(Cfg<NY>::COL_THREADS)
+template <int NY, int V, int PHASE>
+__global__ void __launch_bounds__((ColCfg<NY, V>::THREADS))
 k_col(ColArgs a) {
-  constexpr int E = Cfg<NY>::E, T = Cfg<NY>::T, COLS = Cfg<NY>::COLS;
-  constexpr int NT = Cfg<NY>::COL_THREADS;
+  constexpr int E = ColCfg<NY, V>::E, T = ColCfg<NY, V>::T, COLS = ColCfg<NY, V>::COLS;
+  constexpr int NT = ColCfg<NY, V>::THREADS;
   __shared__ float2 lds[COLS][lds_len<NY>()];
   __shared__ float2 red[16];
   const int ntiles = a.nx / COLS;
@@ -432,6 +455,7 @@ struct bpmh_plan_s {
   size_t ev_used;               // pairs recorded since the last read-out
   // graph mode: the launch sequence of bpmh_propagate is captured once per
   // distinct argument list and replayed with hipGraphLaunch
+  int row_v, col_v;             // launch-shape variants (autotuned)
   int graph_mode;
   hipStream_t cap_stream;
   hipGraphExec_t gexec;
@@ -501,6 +525,8 @@ extern "C" int bpmh_plan_create(int nx, int ny, bpmh_plan* out) {
   p->scratch_bytes = 0;
   p->timing = 0;
   p->ev_used = 0;
+  p->row_v = 1;
+  p->col_v = 1;
   p->graph_mode = 0;
   p->cap_stream = nullptr;
   p->gexec = nullptr;
@@ -619,61 +645,76 @@ extern "C" int bpmh_propagator_table(int nx, int ny, int kind, double dx, double
 typedef void (*row_fn)(const RowArgs&, int batch, hipStream_t);
 typedef void (*col_fn)(const ColArgs&, int batch, hipStream_t);
 
-template <int N>
+template <int N, int V>
 static void launch_row(const RowArgs& a, int batch, hipStream_t s) {
   const int rows = batch * a.ny;
-  hipLaunchKernelGGL(k_row<N>, dim3(rows / Cfg<N>::LINES), dim3(Cfg<N>::ROW_THREADS), 0, s, a);
+  hipLaunchKernelGGL((k_row<N, V>), dim3(rows / RowCfg<N, V>::LINES),
+                     dim3(RowCfg<N, V>::THREADS), 0, s, a);
 }
-template <int N, int PHASE>
+template <int N, int V, int PHASE>
 static void launch_col(const ColArgs& a, int batch, hipStream_t s) {
-  hipLaunchKernelGGL((k_col<N, PHASE>), dim3(a.nx / Cfg<N>::COLS, batch),
-                     dim3(Cfg<N>::COL_THREADS), 0, s, a);
+  hipLaunchKernelGGL((k_col<N, V, PHASE>), dim3(a.nx / ColCfg<N, V>::COLS, batch),
+                     dim3(ColCfg<N, V>::THREADS), 0, s, a);
 }
 
-static row_fn pick_row(int n) {
-  switch (n) {
-    case 16: return launch_row<16>;
-    case 32: return launch_row<32>;
-    case 64: return launch_row<64>;
-    case 128: return launch_row<128>;
-    case 256: return launch_row<256>;
-    case 512: return launch_row<512>;
-    case 1024: return launch_row<1024>;
-    case 2048: return launch_row<2048>;
-    case 4096: return launch_row<4096>;
-  }
+#define BPMH_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+
+static row_fn pick_row(int n, int v) {
+#define ROW_CASE(N)                                    \
+  case N:                                              \
+    return v == 0 ? launch_row<N, 0> : (v == 2 ? launch_row<N, 2> : launch_row<N, 1>);
+  switch (n) { BPMH_SIZES(ROW_CASE) }
+#undef ROW_CASE
   return nullptr;
 }
 
 template <int PHASE>
-static col_fn pick_col(int n) {
-  switch (n) {
-    case 16: return launch_col<16, PHASE>;
-    case 32: return launch_col<32, PHASE>;
-    case 64: return launch_col<64, PHASE>;
-    case 128: return launch_col<128, PHASE>;
-    case 256: return launch_col<256, PHASE>;
-    case 512: return launch_col<512, PHASE>;
-    case 1024: return launch_col<1024, PHASE>;
-    case 2048: return launch_col<2048, PHASE>;
-    case 4096: return launch_col<4096, PHASE>;
-  }
+static col_fn pick_col(int n, int v) {
+#define COL_CASE(N)                                                        \
+  case N:                                                                  \
+    return v == 0 ? launch_col<N, 0, PHASE>                                \
+                  : (v == 2 ? launch_col<N, 2, PHASE> : launch_col<N, 1, PHASE>);
+  switch (n) { BPMH_SIZES(COL_CASE) }
+#undef COL_CASE
   return nullptr;
 }
 
-static int col_tiles(int nx, int ny) {
-  switch (ny) {
-    case 16: return nx / Cfg<16>::COLS;
-    case 32: return nx / Cfg<32>::COLS;
-    case 64: return nx / Cfg<64>::COLS;
-    case 128: return nx / Cfg<128>::COLS;
-    case 256: return nx / Cfg<256>::COLS;
-    case 512: return nx / Cfg<512>::COLS;
-    case 1024: return nx / Cfg<1024>::COLS;
-    case 2048: return nx / Cfg<2048>::COLS;
-    case 4096: return nx / Cfg<4096>::COLS;
-  }
+static int col_tiles(int nx, int ny, int v) {
+#define TILE_CASE(N)                                                              \
+  case N:                                                                         \
+    return nx / (v == 0 ? ColCfg<N, 0>::COLS : (v == 2 ? ColCfg<N, 2>::COLS : ColCfg<N, 1>::COLS));
+  switch (ny) { BPMH_SIZES(TILE_CASE) }
+#undef TILE_CASE
   return 0;
+}
+
+static void variant_shape(int nx, int ny, int rv, int cv, int* lines, int* cols) {
+#define SHAPE_CASE(N)                                                                   \
+  case N:                                                                               \
+    *lines = rv == 0 ? RowCfg<N, 0>::LINES : (rv == 2 ? RowCfg<N, 2>::LINES : RowCfg<N, 1>::LINES); \
+    break;
+  switch (nx) { BPMH_SIZES(SHAPE_CASE) }
+#undef SHAPE_CASE
+  *cols = nx / col_tiles(nx, ny, cv);
+}
+
+extern "C" int bpmh_plan_set_variant(bpmh_plan p, int row_variant, int col_variant) {
+  if (!p) return bpmh_set_error(BPMH_ERR_INVALID, "null plan");
+  if (row_variant < 0 || row_variant > 2 || col_variant < 0 || col_variant > 2)
+    return bpmh_set_error(BPMH_ERR_INVALID, "variants are 0, 1 or 2");
+  p->row_v = row_variant;
+  p->col_v = col_variant;
+  return BPMH_OK;
+}
+
+extern "C" int bpmh_plan_get_variant(bpmh_plan p, int* row_variant, int* col_variant,
+                                     int* lines_per_group, int* cols_per_group) {
+  if (!p || !row_variant || !col_variant || !lines_per_group || !cols_per_group)
+    return bpmh_set_error(BPMH_ERR_INVALID, "null argument");
+  *row_variant = p->row_v;
+  *col_variant = p->col_v;
+  variant_shape(p->nx, p->ny, p->row_v, p->col_v, lines_per_group, cols_per_group);
+  return BPMH_OK;
 }
 
 extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, const void* dn,
@@ -705,11 +746,11 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, 
   const int nx = plan->nx, ny = plan->ny;
   if ((long long)batch * ny > (1LL << 30)) return bpmh_set_error(BPMH_ERR_INVALID, "batch too large");
   hipStream_t s = (hipStream_t)stream;
-  const row_fn row = pick_row(nx);
-  const col_fn col0 = pick_col<0>(ny);
-  const col_fn colA = pick_col<1>(ny);
-  const col_fn colB = pick_col<2>(ny);
-  const int ntiles = col_tiles(nx, ny);
+  const row_fn row = pick_row(nx, plan->row_v);
+  const col_fn col0 = pick_col<0>(ny, plan->col_v);
+  const col_fn colA = pick_col<1>(ny, plan->col_v);
+  const col_fn colB = pick_col<2>(ny, plan->col_v);
+  const int ntiles = col_tiles(nx, ny, plan->col_v);
 
   // scratch: weighted partials (batch*ntiles) + scattering partials (batch*nz*ntiles)
   size_t n_w = (size_t)batch * ntiles;
@@ -780,6 +821,8 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, 
     put(&nz, sizeof nz);
     put(&mean_mode, sizeof mean_mode);
     put(&out_scat, sizeof out_scat);
+    put(&plan->row_v, sizeof plan->row_v);
+    put(&plan->col_v, sizeof plan->col_v);
     if (plan->gexec && key == plan->gkey) {
       HIP_TRY(hipGraphLaunch(plan->gexec, s));
       return BPMH_OK;
@@ -881,8 +924,8 @@ extern "C" int bpmh_adjoint_segment(bpmh_plan plan, int batch, int z_lo, int z_h
   const int nx = plan->nx, ny = plan->ny;
   const int nseg = z_hi - z_lo + 1;
   hipStream_t s = (hipStream_t)stream;
-  const row_fn row = pick_row(nx);
-  const col_fn col0 = pick_col<0>(ny);
+  const row_fn row = pick_row(nx, plan->row_v);
+  const col_fn col0 = pick_col<0>(ny, plan->col_v);
 
   RowArgs ra;
   memset(&ra, 0, sizeof ra);

@@ -72,6 +72,15 @@ int bpmh_plan_kernel_times(bpmh_plan plan, float* ms, int* counts);
  * reference loop (bpm/bpm/bpm_3d.py:283-382).  Ignored while timing is on. */
 int bpmh_plan_set_graph(bpmh_plan plan, int enable);
 
+/* Launch-shape variants of the row and column passes (0, 1, 2: half, the
+ * default, twice the lines / columns per work-group), chosen per grid size
+ * by the autotuner (bpm_amd/tuning.py).  get_variant also reports the lines
+ * per row work-group and columns per column tile the variants map to.  No
+ * reference counterpart (gputools picks its own FFT launch shapes). */
+int bpmh_plan_set_variant(bpmh_plan plan, int row_variant, int col_variant);
+int bpmh_plan_get_variant(bpmh_plan plan, int* row_variant, int* col_variant,
+                          int* lines_per_group, int* cols_per_group);
+
 /* Fill H (ny, nx) complex64 with the angular-spectrum propagator.
  * kind BPMH_H_F32 replaces the compute_propagator kernel launch
  * (bpm/bpm/bpm_3d.py:192-196, bpm_3d_kernels.cl:408-432), all arguments used

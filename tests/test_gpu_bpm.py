@@ -291,3 +291,37 @@ def test_served_results_equal_alone_and_in_mix(bpm):
     # a job that does not fit the budget is refused up front
     with pytest.raises(MemoryError):
         BpmServer(hbm_budget=1000, slots=1).run(jobs[:1])
+
+
+@pytest.mark.parametrize("nx,ny", [(1024, 1024), (256, 512), (4096, 16), (16, 2048)])
+def test_launch_variants_agree(bpm, nx, ny):
+    import torch
+    from bpm_amd import bpm3d as B
+    from bpm_amd import _lib
+    say("case variants", nx, ny)
+    Nz = 6
+    rng = np.random.default_rng(7)
+    dn = torch.from_numpy(rng.uniform(0, .05, (Nz, ny, nx)).astype(np.float32)).cuda()
+    u0 = torch.from_numpy(np.exp(1j * rng.uniform(0, 3, (2, ny, nx))).astype(np.complex64)).cuda()
+    units = (.1, .1, .1)
+    H = B.propagator_table(nx, ny, units, LAM, N0, _lib.H_F64)
+    plan = B.get_plan(nx, ny)
+    lib = _lib.load()
+    outs = {}
+    try:
+        for rv in range(3):
+            for cv in range(3):
+                _lib.check(lib.bpmh_plan_set_variant(plan, rv, cv))
+                for mode in ("ignore", "weighted"):
+                    p = u0.clone()
+                    B.propagate_fields(p, Nz, units, LAM, N0, dn=dn, dn_kind=_lib.DN_F32,
+                                       mean_mode=mode, H=H)
+                    outs[(rv, cv, mode)] = p
+        ref_i, ref_w = outs[(1, 1, "ignore")], outs[(1, 1, "weighted")]
+        for (rv, cv, mode), p in outs.items():
+            if mode == "ignore":
+                assert torch.equal(p, ref_i), (rv, cv)
+            else:  # weighted dn0 sums per-tile partials: tile count changes the order
+                assert ((p - ref_w).abs().max() / ref_w.abs().max()).item() < 1e-5, (rv, cv)
+    finally:
+        _lib.check(lib.bpmh_plan_set_variant(plan, 1, 1))
