@@ -61,7 +61,7 @@ def setup(nz, ny, nx, Bn, seed=0, units=(.1, .1, .1)):
 def test_adjoint_vs_float64_autograd(absorb):
     import torch as t
     from bpm_amd import adjoint
-    nz, ny, nx, Bn = 24, 48, 64, 3
+    nz, ny, nx, Bn = 24, 32, 64, 3
     say("case adjoint vs f64 autograd absorb=%d" % absorb)
     dn, dn2, u0 = setup(nz, ny, nx, Bn, seed=1)
     target = adjoint.propagate(dn2, u0, lam=LAM, n0=N0, absorbing_width=absorb).detach()
