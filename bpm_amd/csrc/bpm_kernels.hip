@@ -81,7 +81,8 @@ struct ColCfg : Base<N> {
   using Base<N>::LDS_LINE;
   static constexpr int C0 = N >= 4096 ? 1 : clampi(512 / T, 2, 16);
   static constexpr int CMAX0 = 131072 / LDS_LINE < 1 ? 1 : 131072 / LDS_LINE;
-  static constexpr int CMAX = pow2floor(clampi(CMAX0 < 1024 / T ? CMAX0 : 1024 / T, 1, 16));
+  static constexpr int TMAX = 512 / T < 1 ? 1 : 512 / T;  // <= 512 threads: no spills
+  static constexpr int CMAX = pow2floor(clampi(CMAX0 < TMAX ? CMAX0 : TMAX, 1, 16));
   static constexpr int COLS = clampi(scale_v(C0, V), 1, CMAX);
   static constexpr int THREADS = COLS * T;
 };
@@ -203,13 +204,36 @@ k_row(RowArgs a) {
   const int b = row / a.ny;
   const int y = row - b * a.ny;
   float2* p = a.plane + (size_t)row * NX;
+  const size_t plane_off = (size_t)y * NX;
 
+  // Every global load of the pass is issued here, before the first barrier,
+  // so their latencies overlap instead of adding up behind the transforms.
   float2 v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = p[t + m * T];
+  Twiddles<NX, E> tw;
+  if (a.inverse || a.forward) tw.load(t, a.tw);
+  float2 d[E];
+  if (a.screen) {
+    const size_t slice = ((size_t)a.z * a.ny + y) * NX + (size_t)b * a.dn_bstride;
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = load_dn(a.dn, a.dn_kind, slice + t + m * T);
+  }
+  float accv[E];
+  float* acc_row = a.acc ? a.acc + (size_t)b * a.ny * NX + plane_off : nullptr;
+  if (acc_row && !a.acc_init) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) accv[m] = acc_row[t + m * T];
+  }
+  float2 ua[E];
+  if (a.adjoint) {
+    const float2* u = a.useg + (size_t)b * a.useg_bstride + (size_t)a.useg_z * a.ny * NX + plane_off;
+#pragma unroll
+    for (int m = 0; m < E; ++m) ua[m] = u[t + m * T];
+  }
 
   if (a.inverse) {
-    line_fft<NX, E, true>(v, lds[line], t, a.tw);
+    line_fft<NX, E, true>(v, lds[line], t, tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = c_scale(v[m], a.inv_n);
   }
@@ -218,29 +242,25 @@ k_row(RowArgs a) {
     float dn0 = 0.f;
     if (a.dn0_mode == BPMH_MEAN_SIMPLE) dn0 = a.dn0_arr[a.z];
     else if (a.dn0_mode == BPMH_MEAN_WEIGHTED) dn0 = weighted_dn0(a.wpart + (size_t)b * a.nparts, a.nparts);
-    const size_t slice = ((size_t)a.z * a.ny + y) * NX + (size_t)b * a.dn_bstride;
     const int disty = min(a.ny - y - 1, y);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int x = t + m * T;
-      const float2 d = load_dn(a.dn, a.dn_kind, slice + x);
       if (a.dn_kind == BPMH_DN_C64) {
         // cexp((0,-k)*(dn - dn0)) = e^{k Im} (cos(-k Re), sin(-k Re)), no taper
-        const float re = f_mul(a.unit_k, d.y);
-        const float im = f_mul(-a.unit_k, f_sub(d.x, dn0));
+        const float re = f_mul(a.unit_k, d[m].y);
+        const float im = f_mul(-a.unit_k, f_sub(d[m].x, dn0));
         float s, c;
-        sincosf(im, &s, &c);
+        sincos_fast(im, &s, &c);
         const float mag = expf(re);
         v[m] = c_mul(v[m], make_float2(mag * c, mag * s));
       } else {
-        const float ph = f_mul(-a.unit_k, f_sub(d.x, dn0));
+        const float ph = f_mul(-a.unit_k, f_sub(d[m].x, dn0));
         float s, c;
-        sincosf(ph, &s, &c);
+        sincos_fast(ph, &s, &c);
         float2 r;
         if (a.adjoint) {
-          const float2 u = a.useg[(size_t)b * a.useg_bstride +
-                                  ((size_t)a.useg_z * a.ny + y) * NX + x];
-          a.gitem[((size_t)b * a.ny + y) * NX + x] = a.unit_k * (v[m].x * u.y - v[m].y * u.x);
+          a.gitem[((size_t)b * a.ny + y) * NX + x] = a.unit_k * (v[m].x * ua[m].y - v[m].y * ua[m].x);
           r = c_mulc(v[m], make_float2(c, s));
         } else {
           r = c_mul(v[m], make_float2(c, s));
@@ -256,7 +276,6 @@ k_row(RowArgs a) {
     }
   }
 
-  const size_t plane_off = (size_t)y * NX;
   if (a.out_field) {
     float2* o = a.out_field + ((size_t)b * a.nz + a.z) * a.ny * NX + plane_off;
 #pragma unroll
@@ -267,16 +286,15 @@ k_row(RowArgs a) {
 #pragma unroll
     for (int m = 0; m < E; ++m) o[t + m * T] = v[m].x * v[m].x + v[m].y * v[m].y;
   }
-  if (a.acc) {
-    float* o = a.acc + (size_t)b * a.ny * NX + plane_off;
+  if (acc_row) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const float i2 = v[m].x * v[m].x + v[m].y * v[m].y;
-      o[t + m * T] = a.acc_init ? i2 : o[t + m * T] + i2;
+      acc_row[t + m * T] = a.acc_init ? i2 : accv[m] + i2;
     }
   }
 
-  if (a.forward) line_fft<NX, E, false>(v, lds[line], t, a.tw);
+  if (a.forward) line_fft<NX, E, false>(v, lds[line], t, tw);
 #pragma unroll
   for (int m = 0; m < E; ++m) p[t + m * T] = v[m];
 }
@@ -323,11 +341,26 @@ k_col(ColArgs a) {
   const int x = tile * COLS + c;
   float2* p = a.plane + (size_t)b * NY * a.nx + x;
 
+  // all global loads first (plane, twiddles, the propagator quadrant)
   float2 v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = p[(size_t)(t + m * T) * a.nx];
+  Twiddles<NY, E> tw;
+  tw.load(t, a.tw);
+  // H depends on kx^2 and ky^2 only, so H[j][i] = H[(ny-j)%ny][(nx-i)%nx]
+  // holds bit for bit; reading the quadrant j<=ny/2, i<=nx/2 keeps the
+  // table's footprint at a quarter (L2-resident per XCD strip).
+  const int qx = min(x, a.nx - x);
+  float2 hv[E];
+  if (PHASE != 1 && a.hmode == 0) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int ky = t + m * T;
+      hv[m] = a.H[(size_t)min(ky, NY - ky) * a.nx + qx];
+    }
+  }
 
-  if constexpr (PHASE != 2) line_fft<NY, E, false>(v, lds[c], t, a.tw);
+  if constexpr (PHASE != 2) line_fft<NY, E, false>(v, lds[c], t, tw);
 
   if constexpr (PHASE == 1) {
     const size_t s0 = (size_t)b * a.dn_bstride;
@@ -350,16 +383,10 @@ k_col(ColArgs a) {
       an = n_term_f32(n, a.k0);
       kx2 = kcomp2_f32(x, a.nx, a.dx);
     }
-    // H depends on kx^2 and ky^2 only, so H[j][i] = H[(ny-j)%ny][(nx-i)%nx]
-    // holds bit for bit; reading the quadrant j<=ny/2, i<=nx/2 keeps the
-    // table's footprint at a quarter (L2-resident per XCD strip).
-    const int qx = min(x, a.nx - x);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int ky = t + m * T;
-      const int qy = min(ky, NY - ky);
-      const float2 h = (a.hmode == 0) ? a.H[(size_t)qy * a.nx + qx]
-                                      : prop_f32(kx2, kcomp2_f32(ky, NY, a.dy), an, a.dz);
+      const float2 h = (a.hmode == 0) ? hv[m] : prop_f32(kx2, kcomp2_f32(ky, NY, a.dy), an, a.dz);
       v[m] = a.conj_h ? c_mulc(v[m], h) : c_mul(v[m], h);
     }
     if (a.sw) {
@@ -381,7 +408,7 @@ k_col(ColArgs a) {
       acc = block_sum2<NT>(acc, red);
       if (threadIdx.x == 0) a.spart[((size_t)b * a.nz + a.z) * ntiles + tile] = acc;
     }
-    line_fft<NY, E, true>(v, lds[c], t, a.tw);
+    line_fft<NY, E, true>(v, lds[c], t, tw);
   }
 
 #pragma unroll

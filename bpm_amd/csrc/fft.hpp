@@ -77,12 +77,14 @@ BPMH_DEV void dft(float2 (&a)[8]) {
   a[3] = c_add(e3, o3); a[7] = c_sub(e3, o3);
 }
 
-// One Stockham stage.  tw[m] = exp(-2 pi i m / N) for m < N.
+// One Stockham stage.  w[r-1] = exp(-2 pi i r k/(NS R)), k = t mod NS, the
+// thread's twiddles of this stage (only radix-E stages, B = 1, have NS > 1).
 template <int N, int E, int R, int NS, bool INV, bool LAST>
 BPMH_DEV void stage(float2 (&v)[E], float2* __restrict__ lds, int t,
-                    const float2* __restrict__ tw) {
+                    const float2* __restrict__ w) {
   constexpr int T = N / E;
   constexpr int B = E / R;
+  static_assert(NS == 1 || B == 1, "twiddled stages are full radix-E stages");
 #pragma unroll
   for (int q = 0; q < B; ++q) {
     const int j = t + q * T;
@@ -92,10 +94,7 @@ BPMH_DEV void stage(float2 (&v)[E], float2* __restrict__ lds, int t,
     for (int r = 0; r < R; ++r) a[r] = v[q + r * B];
     if constexpr (NS > 1) {
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        const float2 w = tw[(r * k) * (N / (NS * R))];
-        a[r] = INV ? c_mulc(a[r], w) : c_mul(a[r], w);
-      }
+      for (int r = 1; r < R; ++r) a[r] = INV ? c_mulc(a[r], w[r - 1]) : c_mul(a[r], w[r - 1]);
     }
     dft<INV>(a);
     if constexpr (LAST) {
@@ -115,12 +114,14 @@ BPMH_DEV void stage(float2 (&v)[E], float2* __restrict__ lds, int t,
   }
 }
 
+// Radix-E stages from stride NS on; w points at this stage's E-1 twiddles
+// (none for NS == 1).
 template <int N, int E, int NS, bool INV>
 struct RadixEStages {
-  BPMH_DEV static void run(float2 (&v)[E], float2* lds, int t, const float2* tw) {
+  BPMH_DEV static void run(float2 (&v)[E], float2* lds, int t, const float2* w) {
     constexpr bool LAST = (NS * E == N);
-    stage<N, E, E, NS, INV, LAST>(v, lds, t, tw);
-    if constexpr (!LAST) RadixEStages<N, E, NS * E, INV>::run(v, lds, t, tw);
+    stage<N, E, E, NS, INV, LAST>(v, lds, t, w);
+    if constexpr (!LAST) RadixEStages<N, E, NS * E, INV>::run(v, lds, t, NS > 1 ? w + (E - 1) : w);
   }
 };
 
@@ -136,18 +137,43 @@ struct Shape {
   static_assert(N >= E * E, "N must hold at least two radix-E stages");
 };
 
+// The twiddles one thread needs for a whole transform, loaded once into
+// registers (they depend on t only) and shared by the forward and inverse
+// transforms of a pass: (E-1) per radix-E stage with Ns > 1.
+template <int N, int E>
+struct Twiddles {
+  using Sh = Shape<N, E>;
+  static constexpr int NS0 = Sh::R0 > 1 ? Sh::R0 : E;   // first stage with Ns > 1
+  static constexpr int NST = Sh::R0 > 1 ? Sh::S : Sh::S - 1;
+  static constexpr int COUNT = NST * (E - 1);
+  float2 w[COUNT > 0 ? COUNT : 1];
+
+  // tw[m] = exp(-2 pi i m/N): the global table of the plan
+  BPMH_DEV void load(int t, const float2* __restrict__ tw) {
+    int ns = NS0;
+#pragma unroll
+    for (int s = 0; s < NST; ++s) {
+      const int k = t & (ns - 1);
+      const int stride = N / (ns * E);
+#pragma unroll
+      for (int r = 1; r < E; ++r) w[s * (E - 1) + r - 1] = tw[r * k * stride];
+      ns *= E;
+    }
+  }
+};
+
 // Unnormalised DFT (INV=false: exp(-2 pi i jk/N); INV=true: exp(+2 pi i jk/N))
 // of the line held as v[m] = x[t + m*T]; result in the same layout.
 // Every thread of the work-group must call it (it synchronises the group).
 template <int N, int E, bool INV>
 BPMH_DEV void line_fft(float2 (&v)[E], float2* __restrict__ lds, int t,
-                       const float2* __restrict__ tw) {
+                       const Twiddles<N, E>& tw) {
   using Sh = Shape<N, E>;
   if constexpr (Sh::R0 > 1) {
-    stage<N, E, Sh::R0, 1, INV, false>(v, lds, t, tw);
-    RadixEStages<N, E, Sh::R0, INV>::run(v, lds, t, tw);
+    stage<N, E, Sh::R0, 1, INV, false>(v, lds, t, nullptr);
+    RadixEStages<N, E, Sh::R0, INV>::run(v, lds, t, tw.w);
   } else {
-    RadixEStages<N, E, 1, INV>::run(v, lds, t, tw);
+    RadixEStages<N, E, 1, INV>::run(v, lds, t, tw.w);
   }
 }
 
