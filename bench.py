@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-debye", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--copy-reference", type=int, default=0, metavar="MIB",
+                    help="only run a MIB-sized device copy (counter scale reference)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the z-loop kernel by kernel instead of replaying a HIP graph")
     return ap.parse_args()
@@ -104,8 +106,24 @@ def served_extra(seed):
             "served_wall_s": r["wall_s"], "plain_wall_s": p["wall_s"]}
 
 
+def copy_reference(mib):
+    """A plain device-to-device copy of known size (the scale for the HBM
+    byte counters FETCH_SIZE / WRITE_SIZE read under rocprofv3 --pmc)."""
+    import torch
+    n = mib << 18  # float32 elements
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_reference_bytes": n * 4, "dispatches": 3}), flush=True)
+
+
 def main():
     a = parse()
+    if a.copy_reference:
+        copy_reference(a.copy_reference)
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(independent jobs would shard by job with no communication); run with --gpus 1",

@@ -138,25 +138,21 @@ BPMH_DEV float weighted_dn0(const float2* __restrict__ wp, int nparts) {
   return sd / su;
 }
 
-// Deterministic block sum of a float2 (wave butterfly, then wave 0 adds the
-// per-wave values in order).  Uses red[] of at least 16 float2 in LDS.
-template <int NT>
-BPMH_DEV float2 block_sum2(float2 v, float2* red) {
+// Sum over the T threads of each column of a tile, in a fixed binary-tree
+// order that depends only on T (not on how many columns share the
+// work-group), so per-column partial sums are identical for every launch
+// shape.  Uses the tile's LDS lines as scratch; returns the sum in t == 0.
+template <int T>
+BPMH_DEV float2 column_sum2(float2 v, float2* __restrict__ col_lds, int t) {
+  __syncthreads();  // the FFT's last reads of col_lds are done
+  col_lds[t] = v;
+  __syncthreads();
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    v.x += __shfl_xor(v.x, o, 64);
-    v.y += __shfl_xor(v.y, o, 64);
+  for (int s = T / 2; s > 0; s >>= 1) {
+    if (t < s) col_lds[t] = c_add(col_lds[t], col_lds[t + s]);
+    __syncthreads();
   }
-  constexpr int NW = (NT + 63) / 64;
-  if constexpr (NW == 1) return v;
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) red[w] = v;
-  __syncthreads();
-  float2 s = red[0];
-#pragma unroll
-  for (int i = 1; i < NW; ++i) s = c_add(s, red[i]);
-  __syncthreads();
-  return s;
+  return col_lds[0];
 }
 
 // ----------------------------------------------------------------------------
@@ -174,7 +170,7 @@ struct RowArgs {
   int absorb;
   int dn0_mode;            // BPMH_MEAN_*: 0 none, 1 per-plane array, 2 weighted
   const float* dn0_arr;    // simple mode: float32 dn0 per plane
-  const float2* wpart;     // weighted: per-tile partials, nparts per item
+  const float2* wpart;     // weighted: per-column partials, nparts (= nx) per item
   int nparts;
   float2* out_field;       // (batch, nz, ny, nx) or null
   float* out_intens;       // (batch, nz, ny, nx) or null
@@ -313,11 +309,11 @@ struct ColArgs {
   const void* dn;          // weighted: slice 0 of each item's dn is the weight
   long long dn_bstride;
   int dn_kind;
-  float2* wpart;           // weighted partials (batch, ntiles)
+  float2* wpart;           // weighted per-column partials (batch, nx)
   const float* sw;         // scattering weights (ny, nx) or null
   const float* gw;
   const float2* pw;        // plane-wave DC per plane (nz)
-  float2* spart;           // scattering partials (batch, nz, ntiles)
+  float2* spart;           // scattering per-column partials (batch, nz, nx)
   int conj_h;              // adjoint: multiply by conj(H)
 };
 
@@ -329,7 +325,6 @@ k_col(ColArgs a) {
   constexpr int E = ColCfg<NY, V>::E, T = ColCfg<NY, V>::T, COLS = ColCfg<NY, V>::COLS;
   constexpr int NT = ColCfg<NY, V>::THREADS;
   __shared__ float2 lds[COLS][lds_len<NY>()];
-  __shared__ float2 red[16];
   const int ntiles = a.nx / COLS;
   const int bx = blockIdx.x;
   // adjacent tiles share cache lines: keep them on one XCD (blocks are
@@ -373,13 +368,13 @@ k_col(ColArgs a) {
       acc.x += au * d;
       acc.y += au;
     }
-    acc = block_sum2<NT>(acc, red);
-    if (threadIdx.x == 0) a.wpart[(size_t)b * ntiles + tile] = acc;
+    acc = column_sum2<T>(acc, lds[c], t);
+    if (t == 0) a.wpart[(size_t)b * a.nx + x] = acc;
   } else {
     float an = 0.f, kx2 = 0.f;
     if (a.hmode != 0) {
       float n = (a.hmode == 1) ? a.n_eff[a.z]
-                               : f_add(a.n0, weighted_dn0(a.wpart + (size_t)b * ntiles, ntiles));
+                               : f_add(a.n0, weighted_dn0(a.wpart + (size_t)b * a.nx, a.nx));
       an = n_term_f32(n, a.k0);
       kx2 = kcomp2_f32(x, a.nx, a.dx);
     }
@@ -405,8 +400,9 @@ k_col(ColArgs a) {
         acc.x += a.sw[k] * a2;
         acc.y += a.gw[k] * a2;
       }
-      acc = block_sum2<NT>(acc, red);
-      if (threadIdx.x == 0) a.spart[((size_t)b * a.nz + a.z) * ntiles + tile] = acc;
+      acc = column_sum2<T>(acc, lds[c], t);
+      if (t == 0) a.spart[((size_t)b * a.nz + a.z) * a.nx + x] = acc;
+      __syncthreads();  // lds[c] is the inverse transform's exchange buffer
     }
     line_fft<NY, E, true>(v, lds[c], t, tw);
   }
@@ -449,14 +445,14 @@ __global__ void k_prop_table(float2* H, int nx, int ny, int kind, double dx, dou
   H[(size_t)j * nx + i] = h;
 }
 
-__global__ void k_scatter_final(const float2* spart, float* out, int nz, int ntiles) {
+__global__ void k_scatter_final(const float2* spart, float* out, int nz, int nparts) {
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   if (z >= nz) return;
   float2 s = make_float2(0.f, 0.f);
   if (z > 0) {
-    const float2* q = spart + ((size_t)b * nz + z) * ntiles;
-    for (int i = 0; i < ntiles; ++i) s = c_add(s, q[i]);
+    const float2* q = spart + ((size_t)b * nz + z) * nparts;
+    for (int i = 0; i < nparts; ++i) s = c_add(s, q[i]);
   }
   out[((size_t)b * nz + z) * 2 + 0] = s.x;
   out[((size_t)b * nz + z) * 2 + 1] = s.y;
@@ -779,9 +775,10 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, 
   const col_fn colB = pick_col<2>(ny, plan->col_v);
   const int ntiles = col_tiles(nx, ny, plan->col_v);
 
-  // scratch: weighted partials (batch*ntiles) + scattering partials (batch*nz*ntiles)
-  size_t n_w = (size_t)batch * ntiles;
-  size_t n_s = out_scat ? (size_t)batch * nz * ntiles : 0;
+  // scratch: weighted per-column partials (batch*nx) + scattering per-column
+  // partials (batch*nz*nx)
+  size_t n_w = (size_t)batch * nx;
+  size_t n_s = out_scat ? (size_t)batch * nz * nx : 0;
   void* scr = nullptr;
   int rc = plan_scratch(plan, (n_w + n_s + 1) * sizeof(float2), &scr);
   if (rc) return rc;
@@ -803,7 +800,7 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, 
   ra.dn0_mode = (dn_kind == BPMH_DN_NONE) ? 0 : mean_mode;
   ra.dn0_arr = dn0;
   ra.wpart = wpart;
-  ra.nparts = ntiles;
+  ra.nparts = nx;
   ra.out_field = out_field;
   ra.out_intens = out_intens;
   ra.acc = out_sum;
@@ -899,7 +896,7 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, 
   }
   if (out_scat) {
     hipLaunchKernelGGL(k_scatter_final, dim3((nz + 63) / 64, batch), dim3(64), 0, s, spart,
-                       out_scat, nz, ntiles);
+                       out_scat, nz, nx);
   }
   if (use_graph) {
     hipGraph_t g = nullptr;

@@ -319,14 +319,9 @@ def test_launch_variants_agree(bpm, nx, ny):
                     B.propagate_fields(p, Nz, units, LAM, N0, dn=dn, dn_kind=_lib.DN_F32,
                                        mean_mode=mode, H=H)
                     outs[(rv, cv, mode)] = p
-        ref_i, ref_w = outs[(1, 1, "ignore")], outs[(1, 1, "weighted")]
+        # per-line arithmetic and the per-column reductions do not depend on
+        # the launch shape, so every variant gives the same bits
         for (rv, cv, mode), p in outs.items():
-            if mode == "ignore":
-                assert torch.equal(p, ref_i), (rv, cv)
-            else:
-                # weighted dn0 sums per-tile partials, so the tile count changes the
-                # summation order; a last-bit change of n0+dn0 can move a component
-                # across the evanescent cut-off, hence an L2 (not max) bar
-                assert ((p - ref_w).norm() / ref_w.norm()).item() < 1e-4, (rv, cv)
+            assert torch.equal(p, outs[(1, 1, mode)]), (rv, cv, mode)
     finally:
         _lib.check(lib.bpmh_plan_set_variant(plan, 1, 1))
