@@ -70,8 +70,11 @@ template <int N, int V>
 struct RowCfg : Base<N> {
   using Base<N>::T;
   static constexpr int L0 = clampi(256 / T, 1, 16);
+  // at least one full wavefront per work-group (the cross-lane reductions
+  // use all 64 lanes), at most 16 lines and 1024 threads
+  static constexpr int LMIN = 64 / T < 1 ? 1 : 64 / T;
   static constexpr int LINES =
-      clampi(clampi(scale_v(L0, V), 1, 16), 1, (1024 / T) < 1 ? 1 : 1024 / T);
+      clampi(clampi(scale_v(L0, V), LMIN, 16), 1, (1024 / T) < 1 ? 1 : 1024 / T);
   static constexpr int THREADS = LINES * T;
 };
 
@@ -83,7 +86,8 @@ struct ColCfg : Base<N> {
   static constexpr int CMAX0 = 131072 / LDS_LINE < 1 ? 1 : 131072 / LDS_LINE;
   static constexpr int TMAX = 512 / T < 1 ? 1 : 512 / T;  // <= 512 threads: no spills
   static constexpr int CMAX = pow2floor(clampi(CMAX0 < TMAX ? CMAX0 : TMAX, 1, 16));
-  static constexpr int COLS = clampi(scale_v(C0, V), 1, CMAX);
+  static constexpr int CMIN = 64 / T < 1 ? 1 : 64 / T;  // >= one full wavefront
+  static constexpr int COLS = clampi(scale_v(C0, V), CMIN, CMAX);
   static constexpr int THREADS = COLS * T;
 };
 
@@ -122,6 +126,7 @@ BPMH_DEV float n_term_f32(float n, float k0) { return f_mul(f_mul(f_mul(n, n), k
 // dn0 = dn_sum/u_sum from per-tile partials (x = dn part, y = u part).
 // Lane p sums partials p, p+64, ...; the xor butterfly then gives every lane
 // (and every wave of every work-group) the bit-identical total.
+// Every work-group that calls it has a multiple of 64 threads.
 BPMH_DEV float weighted_dn0(const float2* __restrict__ wp, int nparts) {
   const int lane = threadIdx.x & 63;
   float sd = 0.f, su = 0.f;
@@ -323,7 +328,6 @@ template <int NY, int V, int PHASE>
 __global__ void __launch_bounds__((ColCfg<NY, V>::THREADS))
 k_col(ColArgs a) {
   constexpr int E = ColCfg<NY, V>::E, T = ColCfg<NY, V>::T, COLS = ColCfg<NY, V>::COLS;
-  constexpr int NT = ColCfg<NY, V>::THREADS;
   __shared__ float2 lds[COLS][lds_len<NY>()];
   const int ntiles = a.nx / COLS;
   const int bx = blockIdx.x;
