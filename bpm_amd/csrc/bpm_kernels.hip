@@ -63,7 +63,9 @@ template <int N>
 struct Base {
   static constexpr int E = (N <= 32) ? 4 : 8;
   static constexpr int T = N / E;
-  static constexpr int LDS_LINE = (N + N / 16) * 8;  // bytes per padded line
+  static constexpr int ROW_SH = 5, COL_SH = 4;   // LDS pad shifts (see fft.hpp)
+  static constexpr int COL_LINE = N + (N >> 4) + 17;  // float2 per column line
+  static constexpr int LDS_LINE = COL_LINE * 8;        // bytes per column line
 };
 
 template <int N, int V>
@@ -198,7 +200,7 @@ template <int NX, int V>
 __global__ void __launch_bounds__((RowCfg<NX, V>::THREADS))
 k_row(RowArgs a) {
   constexpr int E = RowCfg<NX, V>::E, T = RowCfg<NX, V>::T, LINES = RowCfg<NX, V>::LINES;
-  __shared__ float2 lds[LINES][lds_len<NX>()];
+  __shared__ float2 lds[LINES][lds_len<NX, 5>()];
   const int line = threadIdx.x / T;
   const int t = threadIdx.x - line * T;
   const int row = blockIdx.x * LINES + line;  // in [0, batch*ny)
@@ -234,7 +236,7 @@ k_row(RowArgs a) {
   }
 
   if (a.inverse) {
-    line_fft<NX, E, true>(v, lds[line], t, tw);
+    line_fft<NX, E, true, 5>(v, lds[line], t, tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = c_scale(v[m], a.inv_n);
   }
@@ -295,7 +297,7 @@ k_row(RowArgs a) {
     }
   }
 
-  if (a.forward) line_fft<NX, E, false>(v, lds[line], t, tw);
+  if (a.forward) line_fft<NX, E, false, 5>(v, lds[line], t, tw);
 #pragma unroll
   for (int m = 0; m < E; ++m) p[t + m * T] = v[m];
 }
@@ -328,7 +330,9 @@ template <int NY, int V, int PHASE>
 __global__ void __launch_bounds__((ColCfg<NY, V>::THREADS))
 k_col(ColArgs a) {
   constexpr int E = ColCfg<NY, V>::E, T = ColCfg<NY, V>::T, COLS = ColCfg<NY, V>::COLS;
-  __shared__ float2 lds[COLS][lds_len<NY>()];
+  // per-column lines 17 float2 longer than the padded line: the COLS lines a
+  // wavefront touches together then start in different LDS banks
+  __shared__ float2 lds[COLS][ColCfg<NY, V>::COL_LINE];
   const int ntiles = a.nx / COLS;
   const int bx = blockIdx.x;
   // adjacent tiles share cache lines: keep them on one XCD (blocks are
@@ -359,7 +363,7 @@ k_col(ColArgs a) {
     }
   }
 
-  if constexpr (PHASE != 2) line_fft<NY, E, false>(v, lds[c], t, tw);
+  if constexpr (PHASE != 2) line_fft<NY, E, false, 4>(v, lds[c], t, tw);
 
   if constexpr (PHASE == 1) {
     const size_t s0 = (size_t)b * a.dn_bstride;
@@ -408,7 +412,7 @@ k_col(ColArgs a) {
       if (t == 0) a.spart[((size_t)b * a.nz + a.z) * a.nx + x] = acc;
       __syncthreads();  // lds[c] is the inverse transform's exchange buffer
     }
-    line_fft<NY, E, true>(v, lds[c], t, tw);
+    line_fft<NY, E, true, 4>(v, lds[c], t, tw);
   }
 
 #pragma unroll

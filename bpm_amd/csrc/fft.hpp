@@ -21,11 +21,15 @@ namespace bpmfft {
 
 constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 
-// LDS slot for logical index p: one pad slot per 16 to break the bank
-// aliasing of the scattered Stockham writes at small Ns.
-BPMH_DEV int lpad(int p) { return p + (p >> 4); }
-template <int N>
-constexpr int lds_len() { return N + N / 16; }
+// LDS slot for logical index p: one pad slot per 2^SH entries breaks the
+// bank aliasing of the scattered Stockham writes at small Ns (SH = 5 for
+// lines owned by whole wavefronts, SH = 4 for column tiles whose wavefronts
+// interleave several lines; chosen by simulating the bank mapping of every
+// stage for 32- and 64-bank LDS).
+template <int SH>
+BPMH_DEV int lpad(int p) { return p + (p >> SH); }
+template <int N, int SH>
+constexpr int lds_len() { return N + (N >> SH); }
 
 // s * 2^-1/2 with a two-term constant: the float32 rounding of 2^-1/2 alone
 // biases every W8^odd product by -1.7e-8, which compounds over thousands of
@@ -79,7 +83,7 @@ BPMH_DEV void dft(float2 (&a)[8]) {
 
 // One Stockham stage.  w[r-1] = exp(-2 pi i r k/(NS R)), k = t mod NS, the
 // thread's twiddles of this stage (only radix-E stages, B = 1, have NS > 1).
-template <int N, int E, int R, int NS, bool INV, bool LAST>
+template <int N, int E, int R, int NS, bool INV, bool LAST, int SH>
 BPMH_DEV void stage(float2 (&v)[E], float2* __restrict__ lds, int t,
                     const float2* __restrict__ w) {
   constexpr int T = N / E;
@@ -103,25 +107,26 @@ BPMH_DEV void stage(float2 (&v)[E], float2* __restrict__ lds, int t,
     } else {
       const int base = (j - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[lpad(base + r * NS)] = a[r];
+      for (int r = 0; r < R; ++r) lds[lpad<SH>(base + r * NS)] = a[r];
     }
   }
   if constexpr (!LAST) {
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = lds[lpad(t + m * T)];
+    for (int m = 0; m < E; ++m) v[m] = lds[lpad<SH>(t + m * T)];
     __syncthreads();
   }
 }
 
 // Radix-E stages from stride NS on; w points at this stage's E-1 twiddles
 // (none for NS == 1).
-template <int N, int E, int NS, bool INV>
+template <int N, int E, int NS, bool INV, int SH>
 struct RadixEStages {
   BPMH_DEV static void run(float2 (&v)[E], float2* lds, int t, const float2* w) {
     constexpr bool LAST = (NS * E == N);
-    stage<N, E, E, NS, INV, LAST>(v, lds, t, w);
-    if constexpr (!LAST) RadixEStages<N, E, NS * E, INV>::run(v, lds, t, NS > 1 ? w + (E - 1) : w);
+    stage<N, E, E, NS, INV, LAST, SH>(v, lds, t, w);
+    if constexpr (!LAST)
+      RadixEStages<N, E, NS * E, INV, SH>::run(v, lds, t, NS > 1 ? w + (E - 1) : w);
   }
 };
 
@@ -165,15 +170,15 @@ struct Twiddles {
 // Unnormalised DFT (INV=false: exp(-2 pi i jk/N); INV=true: exp(+2 pi i jk/N))
 // of the line held as v[m] = x[t + m*T]; result in the same layout.
 // Every thread of the work-group must call it (it synchronises the group).
-template <int N, int E, bool INV>
+template <int N, int E, bool INV, int SH>
 BPMH_DEV void line_fft(float2 (&v)[E], float2* __restrict__ lds, int t,
                        const Twiddles<N, E>& tw) {
   using Sh = Shape<N, E>;
   if constexpr (Sh::R0 > 1) {
-    stage<N, E, Sh::R0, 1, INV, false>(v, lds, t, nullptr);
-    RadixEStages<N, E, Sh::R0, INV>::run(v, lds, t, tw.w);
+    stage<N, E, Sh::R0, 1, INV, false, SH>(v, lds, t, nullptr);
+    RadixEStages<N, E, Sh::R0, INV, SH>::run(v, lds, t, tw.w);
   } else {
-    RadixEStages<N, E, 1, INV>::run(v, lds, t, tw.w);
+    RadixEStages<N, E, 1, INV, SH>::run(v, lds, t, tw.w);
   }
 }
 
