@@ -11,9 +11,15 @@ into the working plane and runs the whole z-loop.
 
 Prints one JSON line.  value = voxel-steps/s = Nx*Ny*(Nz-1) per step / time.
 roofline: the dominant kernel's algorithmic bytes per launch (row pass:
-plane read+write 16 B + dn 4 B per voxel; column pass: 16 B per voxel)
-divided by its average launch duration from HIP events recorded around
-every launch during the timed region.
+plane read+write 16 B + dn 4 B + sum-intensity read+write 8 B per voxel;
+column pass: plane read+write 16 B per voxel) divided by its average launch
+duration.  That duration comes from two HIP-event measurements on the stream
+the kernels run on: events around the graph-replayed timed region give the
+total, and an eager repeat of the same steps with an event pair around every
+launch gives each kernel's share of it.
+traffic: HBM bytes per launch of that kernel from rocprofv3 FETCH_SIZE and
+WRITE_SIZE runs, read from the committed counter summary when one exists for
+the same kernel and launch shape.
 cpu_baseline: the numpy complex64 restatement of the same loop (the oracle,
 single-threaded numpy FFT) timed on a bounded sample of the workload.
 """
@@ -106,6 +112,32 @@ def served_extra(seed):
             "served_wall_s": r["wall_s"], "plain_wall_s": p["wall_s"]}
 
 
+def plan_variant(lib, plan, kernel):
+    """Launch-shape variant the plan uses for the row (0) or column (1) pass."""
+    import ctypes
+    v = [ctypes.c_int(-1) for _ in range(4)]
+    if lib.bpmh_plan_get_variant(plan, *[ctypes.byref(x) for x in v]) != 0:
+        return -1
+    return v[0].value if kernel == "row_pass" else v[1].value
+
+
+def measured_traffic(kernel, nx, ny, variant):
+    """HBM bytes per launch from the committed counter summary, if it covers
+    this kernel, grid and launch-shape variant.  Its lines read
+    `traffic <kernel> <nx>x<ny> v<variant> <bytes>`."""
+    path = os.path.join(ROOT, "profiles", "bpm_1024x1024_pass_traffic.md")
+    if not os.path.exists(path):
+        return None
+    key = "traffic %s %dx%d v%d " % (kernel, nx, ny, variant)
+    with open(path) as f:
+        for line in f:
+            line = line.strip().strip("`")
+            if line.startswith(key):
+                return {"bytes": float(line[len(key):].split()[0]),
+                        "source": "profiles/bpm_1024x1024_pass_traffic.md"}
+    return None
+
+
 def copy_reference(mib):
     """A plain device-to-device copy of known size (the scale for the HBM
     byte counters FETCH_SIZE / WRITE_SIZE read under rocprofv3 --pmc)."""
@@ -164,11 +196,17 @@ def main():
         step()
     torch.cuda.synchronize()
 
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
+    ev0.record()
     for _ in range(a.steps):
         step()
+    ev1.record()
     torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+    el_wall = time.perf_counter() - t0
+    el = ev0.elapsed_time(ev1) * 1e-3
 
     # per-kernel durations: the same steps again, eagerly launched with a HIP
     # event pair around every kernel (on the stream the kernels run on)
@@ -187,17 +225,23 @@ def main():
     ms_step = el / a.steps * 1e3
     vsteps = Nx * Ny * (Nz - 1)
     value = vsteps / (ms_step * 1e-3)
-    row_avg = ms[0] / max(cnt[0], 1)
-    col_avg = ms[1] / max(cnt[1], 1)
-    row_bytes = (16. + 20. * (Nz - 1)) / Nz * Nx * Ny
+    # bytes per launch: the row pass at z = 0 has no screen (no dn read) and
+    # starts the sum (no read of it); every other launch reads both
+    row_bytes = (24. + 28. * (Nz - 1)) / Nz * Nx * Ny
     col_bytes = 16. * Nx * Ny
-    kern = {"row_pass": {"avg_us": row_avg * 1e3, "launches": cnt[0], "total_ms": ms[0],
-                         "alg_bytes": row_bytes, "GBps": row_bytes / (row_avg * 1e-3) / 1e9},
-            "col_pass": {"avg_us": col_avg * 1e3, "launches": cnt[1], "total_ms": ms[1],
-                         "alg_bytes": col_bytes, "GBps": col_bytes / (col_avg * 1e-3) / 1e9}}
-    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    ev_total = sum(ms[:4])
+    kern = {}
+    for i, (name, nbytes) in enumerate((("row_pass", row_bytes), ("col_pass", col_bytes))):
+        launches = cnt[i] / a.steps  # per step
+        share = ms[i] / ev_total
+        avg_ms = share * ms_step / launches
+        kern[name] = {"avg_us": avg_ms * 1e3, "launches_per_step": launches,
+                      "event_avg_us": ms[i] / cnt[i] * 1e3, "share": share,
+                      "alg_bytes": nbytes, "GBps": nbytes / (avg_ms * 1e-3) / 1e9}
+    dom = max(kern, key=lambda k: kern[k]["share"])
     ach = kern[dom]["GBps"]
-    step_gbs = 36. * vsteps / (ms_step * 1e-3) / 1e9
+    step_gbs = (row_bytes + col_bytes) * (Nz - 1) / (ms_step * 1e-3) / 1e9
+    traffic = measured_traffic(dom, Nx, Ny, plan_variant(lib, plan, dom))
 
     out = {
         "metric": "voxel-steps/s (split-step BPM, 1024^2 grid)",
@@ -220,9 +264,12 @@ def main():
         "hbm_frac_step": step_gbs / HBM_PEAK_GBS,
         "step_effective_GBps": step_gbs,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes"] if traffic else None},
+        "traffic_source": traffic["source"] if traffic else None,
         "kernels": kern,
         "launch": "eager" if a.no_graph else "hip-graph",
+        "ms_per_step_wall": el_wall / a.steps * 1e3,
         "ms_per_step_eager_with_events": el_events / a.steps * 1e3,
     }
     if not a.no_debye:

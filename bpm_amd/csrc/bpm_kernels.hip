@@ -167,7 +167,7 @@ BPMH_DEV float2 column_sum2(float2 v, float2* __restrict__ col_lds, int t) {
 // ----------------------------------------------------------------------------
 struct RowArgs {
   float2* plane;           // (rows = batch*ny, nx)
-  const float2* tw;        // exp(-2 pi i m/nx)
+  const float2* tw;        // stage-major twiddles of the nx-point transform
   const void* dn;          // (nz, ny, nx) per batch item, kind dn_kind
   long long dn_bstride;    // elements between batch items' dn (0: shared)
   int dn_kind;
@@ -307,7 +307,7 @@ k_row(RowArgs a) {
 // ----------------------------------------------------------------------------
 struct ColArgs {
   float2* plane;           // (batch, ny, nx)
-  const float2* tw;        // exp(-2 pi i m/ny)
+  const float2* tw;        // stage-major twiddles of the ny-point transform
   int nx, nz, z;
   int hmode;               // 0: table H, 1: formula with n_eff[z], 2: formula with n0+weighted dn0
   const float2* H;         // (ny, nx)
@@ -543,6 +543,27 @@ static void make_twiddles(int N, std::vector<float2>& tw) {
   }
 }
 
+// Stage-major twiddles for the line FFT of N points with E elements per
+// thread (fft.hpp: Twiddles::load), built from the natural-order table.
+static void make_stage_twiddles(int N, std::vector<float2>& out) {
+  std::vector<float2> full;
+  make_twiddles(N, full);
+  const int E = (N <= 32) ? 4 : 8;  // Base<N>::E
+  int n = 0, e = 0;
+  while ((1 << n) < N) ++n;
+  while ((1 << e) < E) ++e;
+  const int S = n / e, R0 = 1 << (n - S * e);
+  int ns = R0 > 1 ? R0 : E;
+  const int nst = R0 > 1 ? S : S - 1;
+  out.clear();
+  for (int s = 0; s < nst; ++s) {
+    for (int r = 1; r < E; ++r)
+      for (int k = 0; k < ns; ++k) out.push_back(full[(size_t)r * k * (N / (ns * E))]);
+    ns *= E;
+  }
+  if (out.empty()) out.push_back(make_float2(1.f, 0.f));
+}
+
 extern "C" int bpmh_plan_create(int nx, int ny, bpmh_plan* out) {
   if (!out) return bpmh_set_error(BPMH_ERR_INVALID, "null plan pointer");
   *out = nullptr;
@@ -563,15 +584,15 @@ extern "C" int bpmh_plan_create(int nx, int ny, bpmh_plan* out) {
   p->gexec = nullptr;
   (void)hipGetDevice(&p->device);
   std::vector<float2> tx, ty;
-  make_twiddles(nx, tx);
-  make_twiddles(ny, ty);
-  if (hipMalloc(&p->tw_x, nx * sizeof(float2)) != hipSuccess ||
-      hipMalloc(&p->tw_y, ny * sizeof(float2)) != hipSuccess) {
+  make_stage_twiddles(nx, tx);
+  make_stage_twiddles(ny, ty);
+  if (hipMalloc(&p->tw_x, tx.size() * sizeof(float2)) != hipSuccess ||
+      hipMalloc(&p->tw_y, ty.size() * sizeof(float2)) != hipSuccess) {
     delete p;
     return bpmh_set_error(BPMH_ERR_NOMEM, "hipMalloc failed for twiddles");
   }
-  if (hipMemcpy(p->tw_x, tx.data(), nx * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(p->tw_y, ty.data(), ny * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMemcpy(p->tw_x, tx.data(), tx.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p->tw_y, ty.data(), ty.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) {
     (void)hipFree(p->tw_x);
     (void)hipFree(p->tw_y);
     delete p;

@@ -153,15 +153,19 @@ struct Twiddles {
   static constexpr int COUNT = NST * (E - 1);
   float2 w[COUNT > 0 ? COUNT : 1];
 
-  // tw[m] = exp(-2 pi i m/N): the global table of the plan
-  BPMH_DEV void load(int t, const float2* __restrict__ tw) {
-    int ns = NS0;
+  // tab: the plan's stage-major table, for each stage with stride Ns the
+  // entries W_{Ns*E}^{r*k} at [(r-1)*Ns + k] (r = 1..E-1, k < Ns), stages in
+  // order.  One load instruction then reads consecutive k across the
+  // wavefront (a few cache lines) instead of gathering W_N^{r*k*N/(Ns*E)}
+  // from all over a natural-order table.
+  BPMH_DEV void load(int t, const float2* __restrict__ tab) {
+    int ns = NS0, off = 0;
 #pragma unroll
     for (int s = 0; s < NST; ++s) {
       const int k = t & (ns - 1);
-      const int stride = N / (ns * E);
 #pragma unroll
-      for (int r = 1; r < E; ++r) w[s * (E - 1) + r - 1] = tw[r * k * stride];
+      for (int r = 1; r < E; ++r) w[s * (E - 1) + r - 1] = tab[off + (r - 1) * ns + k];
+      off += (E - 1) * ns;
       ns *= E;
     }
   }
