@@ -59,9 +59,12 @@ constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi 
 constexpr int pow2floor(int v) { return v < 2 ? 1 : 2 * pow2floor(v / 2); }
 constexpr int scale_v(int base, int V) { return V == 0 ? base / 2 : (V == 1 ? base : base * 2); }
 
+// elements per thread of an N-point line
+constexpr int line_e(int n) { return n <= 32 ? 4 : 8; }
+
 template <int N>
 struct Base {
-  static constexpr int E = (N <= 32) ? 4 : 8;
+  static constexpr int E = line_e(N);
   static constexpr int T = N / E;
   static constexpr int ROW_SH = 5, COL_SH = 4;   // LDS pad shifts (see fft.hpp)
   static constexpr int COL_LINE = N + (N >> 4) + 17;  // float2 per column line
@@ -200,6 +203,8 @@ template <int NX, int V>
 __global__ void __launch_bounds__((RowCfg<NX, V>::THREADS))
 k_row(RowArgs a) {
   constexpr int E = RowCfg<NX, V>::E, T = RowCfg<NX, V>::T, LINES = RowCfg<NX, V>::LINES;
+  // a line of at most 64 threads lies inside one wavefront (T divides 64)
+  constexpr bool WAVE = T <= 64;
   __shared__ float2 lds[LINES][lds_len<NX, 5>()];
   const int line = threadIdx.x / T;
   const int t = threadIdx.x - line * T;
@@ -213,7 +218,7 @@ k_row(RowArgs a) {
   // so their latencies overlap instead of adding up behind the transforms.
   float2 v[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = p[t + m * T];
+  for (int m = 0; m < E; ++m) v[m] = ld_stream(p + t + m * T);
   Twiddles<NX, E> tw;
   if (a.inverse || a.forward) tw.load(t, a.tw);
   float2 d[E];
@@ -236,7 +241,7 @@ k_row(RowArgs a) {
   }
 
   if (a.inverse) {
-    line_fft<NX, E, true, 5>(v, lds[line], t, tw);
+    line_fft<NX, E, true, 5, WAVE>(v, lds[line], t, tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = c_scale(v[m], a.inv_n);
   }
@@ -282,12 +287,12 @@ k_row(RowArgs a) {
   if (a.out_field) {
     float2* o = a.out_field + ((size_t)b * a.nz + a.z) * a.ny * NX + plane_off;
 #pragma unroll
-    for (int m = 0; m < E; ++m) o[t + m * T] = v[m];
+    for (int m = 0; m < E; ++m) st_stream(o + t + m * T, v[m]);
   }
   if (a.out_intens) {
     float* o = a.out_intens + ((size_t)b * a.nz + a.z) * a.ny * NX + plane_off;
 #pragma unroll
-    for (int m = 0; m < E; ++m) o[t + m * T] = v[m].x * v[m].x + v[m].y * v[m].y;
+    for (int m = 0; m < E; ++m) st_stream(o + t + m * T, v[m].x * v[m].x + v[m].y * v[m].y);
   }
   if (acc_row) {
 #pragma unroll
@@ -297,9 +302,9 @@ k_row(RowArgs a) {
     }
   }
 
-  if (a.forward) line_fft<NX, E, false, 5>(v, lds[line], t, tw);
+  if (a.forward) line_fft<NX, E, false, 5, WAVE>(v, lds[line], t, tw);
 #pragma unroll
-  for (int m = 0; m < E; ++m) p[t + m * T] = v[m];
+  for (int m = 0; m < E; ++m) st_stream(p + t + m * T, v[m]);
 }
 
 // ----------------------------------------------------------------------------
@@ -545,10 +550,9 @@ static void make_twiddles(int N, std::vector<float2>& tw) {
 
 // Stage-major twiddles for the line FFT of N points with E elements per
 // thread (fft.hpp: Twiddles::load), built from the natural-order table.
-static void make_stage_twiddles(int N, std::vector<float2>& out) {
+static void make_stage_twiddles(int N, int E, std::vector<float2>& out) {
   std::vector<float2> full;
   make_twiddles(N, full);
-  const int E = (N <= 32) ? 4 : 8;  // Base<N>::E
   int n = 0, e = 0;
   while ((1 << n) < N) ++n;
   while ((1 << e) < E) ++e;
@@ -584,8 +588,8 @@ extern "C" int bpmh_plan_create(int nx, int ny, bpmh_plan* out) {
   p->gexec = nullptr;
   (void)hipGetDevice(&p->device);
   std::vector<float2> tx, ty;
-  make_stage_twiddles(nx, tx);
-  make_stage_twiddles(ny, ty);
+  make_stage_twiddles(nx, line_e(nx), tx);
+  make_stage_twiddles(ny, line_e(ny), ty);
   if (hipMalloc(&p->tw_x, tx.size() * sizeof(float2)) != hipSuccess ||
       hipMalloc(&p->tw_y, ty.size() * sizeof(float2)) != hipSuccess) {
     delete p;

@@ -62,5 +62,22 @@ BPMH_DEV void sincos_fast(float x, float* s, float* c) {
   }
 }
 
+// Streaming (nontemporal) global accesses.  Used where measurement favours
+// them: the row pass's plane loads and stores (the column tiles that read and
+// write the plane next run on other XCDs, so the lines are not reused from
+// this XCD's L2) and the write-once per-plane output volumes.  The column
+// pass keeps ordinary accesses: neighbouring tiles share 128-B lines in L2.
+typedef float bpmh_f2v __attribute__((ext_vector_type(2)));
+BPMH_DEV float2 ld_stream(const float2* q) {
+  const bpmh_f2v w = __builtin_nontemporal_load((const bpmh_f2v*)q);
+  return make_float2(w.x, w.y);
+}
+BPMH_DEV float ld_stream(const float* q) { return __builtin_nontemporal_load(q); }
+BPMH_DEV void st_stream(float2* q, float2 v) {
+  const bpmh_f2v w = {v.x, v.y};
+  __builtin_nontemporal_store(w, (bpmh_f2v*)q);
+}
+BPMH_DEV void st_stream(float* q, float v) { __builtin_nontemporal_store(v, q); }
+
 // The reference's M_PI in bpm_3d_kernels.cl:3 (a float literal).
 #define BPMH_PI_F 3.14159265358979f

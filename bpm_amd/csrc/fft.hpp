@@ -81,9 +81,25 @@ BPMH_DEV void dft(float2 (&a)[8]) {
   a[3] = c_add(e3, o3); a[7] = c_sub(e3, o3);
 }
 
+// Orders the LDS exchange between the threads of one line.  WAVE: the line's
+// threads are all in one wavefront, whose LDS operations complete in issue
+// order, so a wavefront-scope fence (no s_barrier) suffices and the other
+// wavefronts of the work-group run on undisturbed; otherwise a work-group
+// barrier.
+template <bool WAVE>
+BPMH_DEV void line_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // One Stockham stage.  w[r-1] = exp(-2 pi i r k/(NS R)), k = t mod NS, the
 // thread's twiddles of this stage (only radix-E stages, B = 1, have NS > 1).
-template <int N, int E, int R, int NS, bool INV, bool LAST, int SH>
+template <int N, int E, int R, int NS, bool INV, bool LAST, int SH, bool WAVE>
 BPMH_DEV void stage(float2 (&v)[E], float2* __restrict__ lds, int t,
                     const float2* __restrict__ w) {
   constexpr int T = N / E;
@@ -111,22 +127,22 @@ BPMH_DEV void stage(float2 (&v)[E], float2* __restrict__ lds, int t,
     }
   }
   if constexpr (!LAST) {
-    __syncthreads();
+    line_sync<WAVE>();
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = lds[lpad<SH>(t + m * T)];
-    __syncthreads();
+    line_sync<WAVE>();
   }
 }
 
 // Radix-E stages from stride NS on; w points at this stage's E-1 twiddles
 // (none for NS == 1).
-template <int N, int E, int NS, bool INV, int SH>
+template <int N, int E, int NS, bool INV, int SH, bool WAVE>
 struct RadixEStages {
   BPMH_DEV static void run(float2 (&v)[E], float2* lds, int t, const float2* w) {
     constexpr bool LAST = (NS * E == N);
-    stage<N, E, E, NS, INV, LAST, SH>(v, lds, t, w);
+    stage<N, E, E, NS, INV, LAST, SH, WAVE>(v, lds, t, w);
     if constexpr (!LAST)
-      RadixEStages<N, E, NS * E, INV, SH>::run(v, lds, t, NS > 1 ? w + (E - 1) : w);
+      RadixEStages<N, E, NS * E, INV, SH, WAVE>::run(v, lds, t, NS > 1 ? w + (E - 1) : w);
   }
 };
 
@@ -173,16 +189,18 @@ struct Twiddles {
 
 // Unnormalised DFT (INV=false: exp(-2 pi i jk/N); INV=true: exp(+2 pi i jk/N))
 // of the line held as v[m] = x[t + m*T]; result in the same layout.
-// Every thread of the work-group must call it (it synchronises the group).
-template <int N, int E, bool INV, int SH>
+// Every thread of the work-group must call it (it synchronises the group);
+// WAVE = true when the line's T threads form one wavefront (or part of one)
+// and no other thread touches its LDS line.
+template <int N, int E, bool INV, int SH, bool WAVE = false>
 BPMH_DEV void line_fft(float2 (&v)[E], float2* __restrict__ lds, int t,
                        const Twiddles<N, E>& tw) {
   using Sh = Shape<N, E>;
   if constexpr (Sh::R0 > 1) {
-    stage<N, E, Sh::R0, 1, INV, false, SH>(v, lds, t, nullptr);
-    RadixEStages<N, E, Sh::R0, INV, SH>::run(v, lds, t, tw.w);
+    stage<N, E, Sh::R0, 1, INV, false, SH, WAVE>(v, lds, t, nullptr);
+    RadixEStages<N, E, Sh::R0, INV, SH, WAVE>::run(v, lds, t, tw.w);
   } else {
-    RadixEStages<N, E, 1, INV, SH>::run(v, lds, t, tw.w);
+    RadixEStages<N, E, 1, INV, SH, WAVE>::run(v, lds, t, tw.w);
   }
 }
 
