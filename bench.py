@@ -112,23 +112,24 @@ def served_extra(seed):
             "served_wall_s": r["wall_s"], "plain_wall_s": p["wall_s"]}
 
 
-def plan_variant(lib, plan, kernel):
-    """Launch-shape variant the plan uses for the row (0) or column (1) pass."""
+def plan_shape(lib, plan, kernel):
+    """Launch shape the plan uses: rows per row work-group or columns per
+    column tile, as "lines=<n>" / "cols=<n>"."""
     import ctypes
     v = [ctypes.c_int(-1) for _ in range(4)]
     if lib.bpmh_plan_get_variant(plan, *[ctypes.byref(x) for x in v]) != 0:
-        return -1
-    return v[0].value if kernel == "row_pass" else v[1].value
+        return "?"
+    return "lines=%d" % v[2].value if kernel == "row_pass" else "cols=%d" % v[3].value
 
 
-def measured_traffic(kernel, nx, ny, variant):
+def measured_traffic(kernel, nx, ny, shape):
     """HBM bytes per launch from the committed counter summary, if it covers
-    this kernel, grid and launch-shape variant.  Its lines read
-    `traffic <kernel> <nx>x<ny> v<variant> <bytes>`."""
+    this kernel, grid and launch shape.  Its lines read
+    `traffic <kernel> <nx>x<ny> <shape> <bytes>`."""
     path = os.path.join(ROOT, "profiles", "bpm_1024x1024_pass_traffic.md")
     if not os.path.exists(path):
         return None
-    key = "traffic %s %dx%d v%d " % (kernel, nx, ny, variant)
+    key = "traffic %s %dx%d %s " % (kernel, nx, ny, shape)
     with open(path) as f:
         for line in f:
             line = line.strip().strip("`")
@@ -241,7 +242,7 @@ def main():
     dom = max(kern, key=lambda k: kern[k]["share"])
     ach = kern[dom]["GBps"]
     step_gbs = (row_bytes + col_bytes) * (Nz - 1) / (ms_step * 1e-3) / 1e9
-    traffic = measured_traffic(dom, Nx, Ny, plan_variant(lib, plan, dom))
+    traffic = measured_traffic(dom, Nx, Ny, plan_shape(lib, plan, dom))
 
     out = {
         "metric": "voxel-steps/s (split-step BPM, 1024^2 grid)",

@@ -68,7 +68,7 @@ def autotune(nx, ny, batch=1, nz=65, reps=3, units=(.1, .1, .1), lam=.5, n0=1.33
         for cv in range(3):
             _lib.call("bpmh_plan_set_variant", plan, rv, cv)
             shp = shape_of(plan)
-            key = (shp[2], shp[3])
+            key = [shp[2], shp[3]]
             if key in [v["shape"] for v in table.values()]:
                 continue  # clamped to an identical launch shape
             def run():
@@ -84,7 +84,7 @@ def autotune(nx, ny, batch=1, nz=65, reps=3, units=(.1, .1, .1), lam=.5, n0=1.33
             e1.record()
             t.cuda.synchronize()
             ms = e0.elapsed_time(e1) / reps
-            table["%d,%d" % (rv, cv)] = {"ms": ms, "shape": list(key),
+            table["%d,%d" % (rv, cv)] = {"ms": ms, "shape": key,
                                          "us_per_step": ms * 1e3 / (nz - 1)}
     best = min(table, key=lambda k: table[k]["ms"])
     bv = tuple(int(x) for x in best.split(","))
