@@ -63,16 +63,23 @@ constexpr int scale_v(int base, int V) { return V == 0 ? base / 2 : (V == 1 ? ba
 // wavefronts of E = 8 for the same data, so loads, transforms and stores of
 // different wavefronts overlap better; measured 5-13 % faster per z-step at
 // 64^2..1024^2, batch 1), 8 above.  Columns: 8 (4-column tiles of 8-element
-// threads; E = 4 measured slower at 1024^2 and at 256^2..512^2 in batches).
+// threads; E = 4 measured slower at 1024^2, also with 4-column tiles of
+// 1024 threads, and at 256^2..512^2 in batches).
 constexpr int row_e(int n) { return n <= 1024 ? 4 : 8; }
 constexpr int col_e(int n) { return n <= 32 ? 4 : 8; }
+#ifndef BPMH_COL_SH
+#define BPMH_COL_SH 5
+#endif
+#ifndef BPMH_COL_SKEW
+#define BPMH_COL_SKEW 9
+#endif
 
 template <int N, int E_>
 struct Base {
   static constexpr int E = E_;
   static constexpr int T = N / E;
-  static constexpr int ROW_SH = 5, COL_SH = 4;   // LDS pad shifts (see fft.hpp)
-  static constexpr int COL_LINE = N + (N >> 4) + 17;  // float2 per column line
+  static constexpr int ROW_SH = 5, COL_SH = BPMH_COL_SH;   // LDS pad shifts (see fft.hpp)
+  static constexpr int COL_LINE = N + (N >> BPMH_COL_SH) + BPMH_COL_SKEW;  // float2 per column line
   static constexpr int LDS_LINE = COL_LINE * 8;        // bytes per column line
 };
 
@@ -375,7 +382,7 @@ k_col(ColArgs a) {
     }
   }
 
-  if constexpr (PHASE != 2) line_fft<NY, E, false, 4>(v, lds[c], t, tw);
+  if constexpr (PHASE != 2) line_fft<NY, E, false, BPMH_COL_SH>(v, lds[c], t, tw);
 
   if constexpr (PHASE == 1) {
     const size_t s0 = (size_t)b * a.dn_bstride;
@@ -424,7 +431,7 @@ k_col(ColArgs a) {
       if (t == 0) a.spart[((size_t)b * a.nz + a.z) * a.nx + x] = acc;
       __syncthreads();  // lds[c] is the inverse transform's exchange buffer
     }
-    line_fft<NY, E, true, 4>(v, lds[c], t, tw);
+    line_fft<NY, E, true, BPMH_COL_SH>(v, lds[c], t, tw);
   }
 
 #pragma unroll
