@@ -67,19 +67,19 @@ constexpr int scale_v(int base, int V) { return V == 0 ? base / 2 : (V == 1 ? ba
 // 1024 threads, and at 256^2..512^2 in batches).
 constexpr int row_e(int n) { return n <= 1024 ? 4 : 8; }
 constexpr int col_e(int n) { return n <= 32 ? 4 : 8; }
-#ifndef BPMH_COL_SH
-#define BPMH_COL_SH 5
-#endif
-#ifndef BPMH_COL_SKEW
-#define BPMH_COL_SKEW 9
-#endif
+
 
 template <int N, int E_>
 struct Base {
   static constexpr int E = E_;
   static constexpr int T = N / E;
-  static constexpr int ROW_SH = 5, COL_SH = BPMH_COL_SH;   // LDS pad shifts (see fft.hpp)
-  static constexpr int COL_LINE = N + (N >> BPMH_COL_SH) + BPMH_COL_SKEW;  // float2 per column line
+  // LDS layouts, from sweeps of the real alternating loop at 1024^2: one
+  // pad slot per 32 entries for row lines and column lines, and column lines
+  // 9 float2 longer than padded, so the 4 columns a wavefront touches start
+  // in different banks (a 17-slot skew with one pad per 16 was 1.3 us per
+  // z-step slower; row pads of one per 4..64 are within 2 %).
+  static constexpr int ROW_SH = 5, COL_SH = 5;
+  static constexpr int COL_LINE = N + (N >> COL_SH) + 9;  // float2 per column line
   static constexpr int LDS_LINE = COL_LINE * 8;        // bytes per column line
 };
 
@@ -219,7 +219,7 @@ k_row(RowArgs a) {
   constexpr int E = RowCfg<NX, V>::E, T = RowCfg<NX, V>::T, LINES = RowCfg<NX, V>::LINES;
   // a line of at most 64 threads lies inside one wavefront (T divides 64)
   constexpr bool WAVE = T <= 64;
-  __shared__ float2 lds[LINES][lds_len<NX, 5>()];
+  __shared__ float2 lds[LINES][lds_len<NX, RowCfg<NX, V>::ROW_SH>()];
   const int line = threadIdx.x / T;
   const int t = threadIdx.x - line * T;
   const int row = blockIdx.x * LINES + line;  // in [0, batch*ny)
@@ -255,7 +255,7 @@ k_row(RowArgs a) {
   }
 
   if (a.inverse) {
-    line_fft<NX, E, true, 5, WAVE>(v, lds[line], t, tw);
+    line_fft<NX, E, true, RowCfg<NX, V>::ROW_SH, WAVE>(v, lds[line], t, tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = c_scale(v[m], a.inv_n);
   }
@@ -287,7 +287,7 @@ k_row(RowArgs a) {
         } else {
           r = c_mul(v[m], make_float2(c, s));
         }
-        const int dist = min(min(NX - x - 1, x), disty);
+        const int dist = a.absorb > 0 ? min(min(NX - x - 1, x), disty) : 0;
         if (dist < a.absorb) {
           const float arg = f_div(f_mul(BPMH_PI_F, (float)dist), (float)a.absorb);
           const float tp = f_mul(0.5f, f_sub(1.f, (float)cos((double)arg)));
@@ -316,7 +316,7 @@ k_row(RowArgs a) {
     }
   }
 
-  if (a.forward) line_fft<NX, E, false, 5, WAVE>(v, lds[line], t, tw);
+  if (a.forward) line_fft<NX, E, false, RowCfg<NX, V>::ROW_SH, WAVE>(v, lds[line], t, tw);
 #pragma unroll
   for (int m = 0; m < E; ++m) st_stream(p + t + m * T, v[m]);
 }
@@ -382,7 +382,7 @@ k_col(ColArgs a) {
     }
   }
 
-  if constexpr (PHASE != 2) line_fft<NY, E, false, BPMH_COL_SH>(v, lds[c], t, tw);
+  if constexpr (PHASE != 2) line_fft<NY, E, false, ColCfg<NY, V>::COL_SH>(v, lds[c], t, tw);
 
   if constexpr (PHASE == 1) {
     const size_t s0 = (size_t)b * a.dn_bstride;
@@ -431,7 +431,7 @@ k_col(ColArgs a) {
       if (t == 0) a.spart[((size_t)b * a.nz + a.z) * a.nx + x] = acc;
       __syncthreads();  // lds[c] is the inverse transform's exchange buffer
     }
-    line_fft<NY, E, true, BPMH_COL_SH>(v, lds[c], t, tw);
+    line_fft<NY, E, true, ColCfg<NY, V>::COL_SH>(v, lds[c], t, tw);
   }
 
 #pragma unroll

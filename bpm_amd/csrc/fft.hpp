@@ -22,10 +22,8 @@ namespace bpmfft {
 constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 
 // LDS slot for logical index p: one pad slot per 2^SH entries breaks the
-// bank aliasing of the scattered Stockham writes at small Ns (SH = 5 for
-// lines owned by whole wavefronts, SH = 4 for column tiles whose wavefronts
-// interleave several lines; chosen by simulating the bank mapping of every
-// stage for 32- and 64-bank LDS).
+// bank aliasing of the scattered Stockham writes at small Ns (the callers
+// choose SH by measurement).
 template <int SH>
 BPMH_DEV int lpad(int p) { return p + (p >> SH); }
 template <int N, int SH>
