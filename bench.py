@@ -20,8 +20,9 @@ launch gives each kernel's share of it.
 traffic: HBM bytes per launch of that kernel from rocprofv3 FETCH_SIZE and
 WRITE_SIZE runs, read from the committed counter summary when one exists for
 the same kernel and launch shape.
-cpu_baseline: the numpy complex64 restatement of the same loop (the oracle,
-single-threaded numpy FFT) timed on a bounded sample of the workload.
+cpu_baseline: the numpy complex64 restatement of the same loop (the oracle)
+timed on a bounded sample of the workload, on all usable host cores (2-D FFTs
+through scipy.fft workers) and on one core.
 """
 import argparse
 import json
@@ -36,6 +37,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector
 
 
 def parse():
@@ -46,8 +48,10 @@ def parse():
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--nz", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-sample-planes", type=int, default=65,
-                    help="planes of the workload the CPU baseline propagates")
+    ap.add_argument("--cpu-sample-planes", type=int, default=129,
+                    help="planes of the workload the all-cores CPU baseline propagates")
+    ap.add_argument("--cpu-sample-planes-1core", type=int, default=65,
+                    help="planes of the workload the one-core CPU baseline propagates")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-debye", action="store_true")
     ap.add_argument("--no-served", action="store_true")
@@ -58,19 +62,45 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(u0, dn, units, lam, n0, planes):
-    """Time the oracle's complex64 restatement of the Bpm3d loop on the host."""
+def host_cores():
+    """Cores this process may use (affinity), capped by OMP_NUM_THREADS when set."""
+    n = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        n = min(n, int(os.environ["OMP_NUM_THREADS"]))
+    return max(1, n)
+
+
+def cpu_baseline(u0, dn, units, lam, n0, planes, planes_1core):
+    """Time the oracle's complex64 restatement of the Bpm3d loop on the host:
+    on all usable cores (2-D FFTs through scipy.fft workers; the element-wise
+    steps are numpy, single-threaded) and on one core (numpy.fft)."""
     from oracle import bpm_oracle as O
     Ny, Nx = u0.shape
-    sub = np.ascontiguousarray(dn[:planes])
-    t0 = time.perf_counter()
-    O.propagate_single((Nx, Ny, planes), units, lam, n0, sub, u0, return_full=False)
-    dt = time.perf_counter() - t0
-    vs = Nx * Ny * (planes - 1) / dt
-    return {"value": vs, "unit": "voxel-steps/s", "cores": 1, "kind": "port",
-            "sample": "%dx%d grid x %d planes (%d steps) of the same workload, numpy "
-                      "complex64 restatement (oracle.bpm_oracle.propagate_single), %.1f s"
-                      % (Nx, Ny, planes, planes - 1, dt)}
+    out = {}
+    for cores, pl in ((host_cores(), planes), (1, planes_1core)):
+        sub = np.ascontiguousarray(dn[:pl])
+        O.set_fft_workers(cores)
+        # untimed warm-up: imports, FFT plans and thread pools
+        O.propagate_single((Nx, Ny, 2), units, lam, n0, sub[:2], u0, return_full=False)
+        t0 = time.perf_counter()
+        O.propagate_single((Nx, Ny, pl), units, lam, n0, sub, u0, return_full=False)
+        dt = time.perf_counter() - t0
+        O.set_fft_workers(1)
+        out[cores] = (Nx * Ny * (pl - 1) / dt, dt, pl)
+    n = host_cores()
+    vs, dt, pl = out[n]
+    vs1, dt1, pl1 = out[1]
+    try:
+        model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+    except (OSError, IndexError):
+        model = "unknown"
+    return {"value": vs, "unit": "voxel-steps/s", "cores": n, "kind": "port",
+            "sample": "%dx%d grid x %d planes (%d steps) of the same workload, the oracle's "
+                      "complex64 restatement (oracle.bpm_oracle.propagate_single) with "
+                      "scipy.fft on %d workers, %.1f s; CPU %s"
+                      % (Nx, Ny, pl, pl - 1, n, dt, model),
+            "single_core_value": vs1,
+            "single_core_sample": "%d planes, numpy.fft, %.1f s" % (pl1, dt1)}
 
 
 def debye_extra(torch):
@@ -87,9 +117,14 @@ def debye_extra(torch):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / reps
     samples = 129 ** 3 * (S + 1)
+    # 6 fused multiply-adds (12 flop) per theta-sample and voxel in the
+    # product kernel; the per-sample Bessel and phase tables are not counted
+    tflops = samples * 12 / dt / 1e12
     return {"workload": "Debye-Wolf 256^3, NA 1.1, n0 1.33, 512 theta steps, x-polarised",
             "ms": dt * 1e3, "theta_samples_per_s": samples / dt,
-            "note": "octant quadrature + full-grid mirror, outputs on device"}
+            "tflops_12_per_sample": tflops, "frac_fp32_vector_peak": tflops / FP32_PEAK_TFLOPS,
+            "note": "octant quadrature (sample/phase tables + tiled product) + full-grid "
+                    "mirror, outputs on device"}
 
 
 def served_extra(seed):
@@ -280,8 +315,9 @@ def main():
     if not a.no_cpu_baseline:
         from bpm_amd.workloads import spheres_dn
         planes = min(a.cpu_sample_planes, Nz)
-        dn_s = spheres_dn((Nx, Ny, planes), units, *sph)
-        out["cpu_baseline"] = cpu_baseline(u0, dn_s, units, lam, n0, planes)
+        planes1 = min(a.cpu_sample_planes_1core, Nz)
+        dn_s = spheres_dn((Nx, Ny, max(planes, planes1)), units, *sph)
+        out["cpu_baseline"] = cpu_baseline(u0, dn_s, units, lam, n0, planes, planes1)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 

@@ -6,14 +6,16 @@
 //   I0 += w(t) (cos t + 1) J0(kr sin t) e^{i kz cos t}
 //   I1 += w(t)  sin t     J1(kr sin t) e^{i kz cos t}
 //   I2 += w(t) (cos t - 1) J2(kr sin t) e^{i kz cos t}
-// and combine them into Ex, Ey, Ez and I.  Everything except the phase
-// e^{i kz cos t} is independent of z, so a work-group takes one (x, y)
-// column: it first tabulates, per theta sample, cos t and the three real
-// amplitudes (Bessel functions evaluated once per column in float64) in LDS,
-// then each lane accumulates the phase sums for its z values.  The column's
-// 3x(S+1) Bessel evaluations are thus shared by all Nz voxels instead of
-// being recomputed per voxel.  Samples are processed in LDS chunks of
-// CHUNK so any number of integration steps fits.
+// and combine them into Ex, Ey, Ez and I.  The phase factor depends on
+// (z, t) only and the real amplitudes on (x, y, t) only, so for the whole
+// grid
+//   I_n[z][col] = sum_s P[z][s] A_n[s][col],   P = e^{i kz cos t}
+// is a product of a (Nz x S') complex and an (S' x NxNy) real matrix.  The
+// grid path therefore tabulates A_n (Bessel functions in float64, once per
+// sample and column) and P (one sincos per sample and z, instead of one per
+// sample and voxel), then runs a register-tiled product whose epilogue
+// combines the three sums into the outputs.  Every output sums its samples
+// in index order with one fused multiply-add each, as the per-voxel form.
 #include "common.hpp"
 #include "bessel.hpp"
 #include "bpm_hip.h"
@@ -77,11 +79,13 @@ BPMH_DEV Sample make_sample(const DwArgs& a, int i_n, int i_t, float kr) {
     w = f_mul(w, rhs);
   }
   const double ks = (double)f_mul(kr, si);
+  double j0, j1, j2;
+  bpmbessel::j012(ks, j0, j1, j2);
   Sample s;
   s.co = co;
-  s.a0 = f_mul(f_mul(w, f_add(co, 1.f)), (float)bpmbessel::jn(0, ks));
-  s.a1 = f_mul(f_mul(w, si), (float)bpmbessel::jn(1, ks));
-  s.a2 = f_mul(f_mul(w, f_sub(co, 1.f)), (float)bpmbessel::jn(2, ks));
+  s.a0 = f_mul(f_mul(w, f_add(co, 1.f)), (float)j0);
+  s.a1 = f_mul(f_mul(w, si), (float)j1);
+  s.a2 = f_mul(f_mul(w, f_sub(co, 1.f)), (float)j2);
   return s;
 }
 
@@ -109,68 +113,129 @@ BPMH_DEV void combine(const DwArgs& a, float phi, float2 I0, float2 I1, float2 I
   }
 }
 
-constexpr int DW_THREADS = 64;   // one wavefront per (x, y) column
-constexpr int DW_ZPT = 4;        // z values per lane per sweep
-constexpr int DW_CHUNK = 512;    // theta samples per LDS chunk
-
-__global__ void __launch_bounds__(DW_THREADS) k_debye_grid(DwArgs a) {
-  __shared__ float s_co[DW_CHUNK], s_a0[DW_CHUNK], s_a1[DW_CHUNK], s_a2[DW_CHUNK];
-  const int col = blockIdx.x;
-  const int i = col % a.nx;
-  const int j = col / a.nx;
+// ----------------------------------------------------------------------------
+// grid path: sample table, phase table, tiled product
+// ----------------------------------------------------------------------------
+// co[s] = cos t_s and A[n][s][col] = the three real amplitudes of sample s at
+// lateral point col = j*nx + i.
+__global__ void __launch_bounds__(256) k_dw_samples(DwArgs a, int total, int ncols, float* co, float* A) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  if (col >= ncols) return;
+  const int i = col % a.nx, j = col / a.nx;
   const float x = axis_coord(i, a.nx, a.x1, a.x2);
   const float y = axis_coord(j, a.ny, a.y1, a.y2);
   const float kr = k_times(a.lam, __fsqrt_rn(f_add(f_mul(x, x), f_mul(y, y))));
-  const float phi = (float)atan2((double)y, (double)x);
   const int per_ann = a.S + 1;
-  const int total = per_ann * (a.n_alphas / 2);
-  const size_t plane = (size_t)a.nx * a.ny;
+  const Sample sm = make_sample(a, s / per_ann, s % per_ann, kr);
+  const size_t plane = (size_t)total * ncols;
+  const size_t o = (size_t)s * ncols + col;
+  A[o] = sm.a0;
+  A[plane + o] = sm.a1;
+  A[2 * plane + o] = sm.a2;
+  if (col == 0) co[s] = sm.co;
+}
 
-  for (int z0 = 0; z0 < a.nz; z0 += DW_THREADS * DW_ZPT) {
-    float kz[DW_ZPT];
-    float2 I0[DW_ZPT], I1[DW_ZPT], I2[DW_ZPT];
+// P[s][z] = e^{i kz cos t_s}, with the float32 argument kz*cos t
+__global__ void __launch_bounds__(64) k_dw_phases(DwArgs a, int total, const float* co, float2* P) {
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  if (z >= a.nz || s >= total) return;
+  const float kz = k_times(a.lam, axis_coord(z, a.nz, a.z1, a.z2));
+  float sn, cs;
+  sincosf(kz * co[s], &sn, &cs);
+  P[(size_t)s * a.nz + z] = make_float2(cs, sn);
+}
+
+// Work-group tile: BZ = 4*TZ planes x DG_BC lateral points; thread (tz, tc)
+// owns planes z0 + 4 tz + {0..3} and points c0 + tc + 16 {0..3}; samples
+// are staged in LDS DG_KS at a time.
+constexpr int DG_BC = 64;
+constexpr int DG_KS = 16;
+
+template <int TZ>
+__global__ void __launch_bounds__(TZ * 16) k_dw_product(DwArgs a, int total, int ncols,
+                                                        const float2* __restrict__ P,
+                                                        const float* __restrict__ A) {
+  constexpr int BZ = 4 * TZ, NT = TZ * 16;
+  __shared__ float2 sP[DG_KS][BZ];
+  __shared__ float sA[3][DG_KS][DG_BC];
+  const int c0 = blockIdx.x * DG_BC, z0 = blockIdx.y * BZ;
+  const int tid = threadIdx.x, tc = tid & 15, tz = tid >> 4;
+  const size_t plane = (size_t)total * ncols;
+  float2 acc[3][4][4];
 #pragma unroll
-    for (int q = 0; q < DW_ZPT; ++q) {
-      const int k = z0 + threadIdx.x + q * DW_THREADS;
-      kz[q] = k < a.nz ? k_times(a.lam, axis_coord(k, a.nz, a.z1, a.z2)) : 0.f;
-      I0[q] = I1[q] = I2[q] = make_float2(0.f, 0.f);
+  for (int n = 0; n < 3; ++n)
+#pragma unroll
+    for (int zi = 0; zi < 4; ++zi)
+#pragma unroll
+      for (int ci = 0; ci < 4; ++ci) acc[n][zi][ci] = make_float2(0.f, 0.f);
+
+  for (int s0 = 0; s0 < total; s0 += DG_KS) {
+    __syncthreads();
+    for (int e = tid; e < DG_KS * BZ; e += NT) {
+      const int ks = e / BZ, zz = e - ks * BZ;
+      const int s = s0 + ks, z = z0 + zz;
+      sP[ks][zz] = (s < total && z < a.nz) ? P[(size_t)s * a.nz + z] : make_float2(0.f, 0.f);
     }
-    const int nq = min(DW_ZPT, (a.nz - z0 + DW_THREADS - 1) / DW_THREADS);
-    for (int c0 = 0; c0 < total; c0 += DW_CHUNK) {
-      const int cn = min(DW_CHUNK, total - c0);
-      __syncthreads();
-      for (int s = threadIdx.x; s < cn; s += DW_THREADS) {
-        const int g = c0 + s;
-        const Sample sm = make_sample(a, g / per_ann, g % per_ann, kr);
-        s_co[s] = sm.co;
-        s_a0[s] = sm.a0;
-        s_a1[s] = sm.a1;
-        s_a2[s] = sm.a2;
-      }
-      __syncthreads();
-      for (int s = 0; s < cn; ++s) {
-        const float co = s_co[s], r0 = s_a0[s], r1 = s_a1[s], r2 = s_a2[s];
+    for (int e = tid; e < 3 * DG_KS * DG_BC; e += NT) {
+      const int n = e / (DG_KS * DG_BC), r = e - n * (DG_KS * DG_BC);
+      const int ks = r / DG_BC, cc = r - ks * DG_BC;
+      const int s = s0 + ks, c = c0 + cc;
+      sA[n][ks][cc] = (s < total && c < ncols) ? A[n * plane + (size_t)s * ncols + c] : 0.f;
+    }
+    __syncthreads();
+    const int kn = min(DG_KS, total - s0);
+    for (int ks = 0; ks < kn; ++ks) {
+      float2 p[4];
+      float w[3][4];
 #pragma unroll
-        for (int q = 0; q < DW_ZPT; ++q) {
-          if (q < nq) {
-            float sn, cs;
-            sincosf(kz[q] * co, &sn, &cs);
-            I0[q].x += r0 * cs; I0[q].y += r0 * sn;
-            I1[q].x += r1 * cs; I1[q].y += r1 * sn;
-            I2[q].x += r2 * cs; I2[q].y += r2 * sn;
+      for (int zi = 0; zi < 4; ++zi) p[zi] = sP[ks][4 * tz + zi];
+#pragma unroll
+      for (int n = 0; n < 3; ++n)
+#pragma unroll
+        for (int ci = 0; ci < 4; ++ci) w[n][ci] = sA[n][ks][tc + 16 * ci];
+#pragma unroll
+      for (int n = 0; n < 3; ++n)
+#pragma unroll
+        for (int zi = 0; zi < 4; ++zi)
+#pragma unroll
+          for (int ci = 0; ci < 4; ++ci) {
+            acc[n][zi][ci].x = fmaf(w[n][ci], p[zi].x, acc[n][zi][ci].x);
+            acc[n][zi][ci].y = fmaf(w[n][ci], p[zi].y, acc[n][zi][ci].y);
           }
-        }
-      }
     }
+  }
+
+  const size_t out_plane = (size_t)a.nx * a.ny;
 #pragma unroll
-    for (int q = 0; q < DW_ZPT; ++q) {
-      const int k = z0 + threadIdx.x + q * DW_THREADS;
-      if (k < a.nz) combine(a, phi, I0[q], I1[q], I2[q], (size_t)k * plane + col);
+  for (int ci = 0; ci < 4; ++ci) {
+    const int c = c0 + tc + 16 * ci;
+    if (c >= ncols) continue;
+    const int i = c % a.nx, j = c / a.nx;
+    const float x = axis_coord(i, a.nx, a.x1, a.x2);
+    const float y = axis_coord(j, a.ny, a.y1, a.y2);
+    const float phi = (float)atan2((double)y, (double)x);
+#pragma unroll
+    for (int zi = 0; zi < 4; ++zi) {
+      const int z = z0 + 4 * tz + zi;
+      if (z < a.nz) combine(a, phi, acc[0][zi][ci], acc[1][zi][ci], acc[2][zi][ci], (size_t)z * out_plane + c);
     }
   }
 }
 
-__global__ void k_debye_points(DwArgs a, int n) {
+// planes per work-group tile: the z-tile size with the least padded work
+static int pick_tz(int nz) {
+  int best = 16;
+  long long best_work = -1;
+  for (int tz : {16, 12, 8, 4}) {
+    const long long work = (long long)((nz + 4 * tz - 1) / (4 * tz)) * 4 * tz;
+    if (best_work < 0 || work < best_work) { best = tz; best_work = work; }
+  }
+  return best;
+}
+
+__global__ void __launch_bounds__(64) k_debye_points(DwArgs a, int n) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const float x = a.px[p], y = a.py[p], z = a.pz[p];
@@ -258,8 +323,31 @@ extern "C" int bpmh_debye_wolf(int nx, int ny, int nz, float x1, float x2, float
   a.nx = nx; a.ny = ny; a.nz = nz;
   a.x1 = x1; a.x2 = x2; a.y1 = y1; a.y2 = y2; a.z1 = z1; a.z2 = z2;
   a.ex = (float2*)ex; a.ey = (float2*)ey; a.ez = (float2*)ez; a.I = I;
-  hipLaunchKernelGGL(k_debye_grid, dim3(nx * ny), dim3(DW_THREADS), 0, (hipStream_t)stream, a);
-  hipError_t e = hipGetLastError();
+  hipStream_t st = (hipStream_t)stream;
+  const int ncols = nx * ny;
+  const int total = (int_steps + 1) * (n_alphas / 2);
+  if ((long long)ncols * total > (1LL << 31) || (long long)nz * total > (1LL << 31) || total > 65535)
+    return bpmh_set_error(BPMH_ERR_INVALID, "grid x samples too large");
+  // stream-ordered scratch: A (3 x total x ncols), co (total), P (total x nz)
+  const size_t nA = (size_t)3 * total * ncols, nP = (size_t)total * nz;
+  void* scr = nullptr;
+  hipError_t e = hipMallocAsync(&scr, nA * sizeof(float) + nP * sizeof(float2) + total * sizeof(float) + 256, st);
+  if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_NOMEM, "hipMallocAsync failed for the Debye tables");
+  float* A = (float*)scr;
+  float2* P = (float2*)(((uintptr_t)(A + nA) + 15) & ~(uintptr_t)15);
+  float* co = (float*)(P + nP);
+  hipLaunchKernelGGL(k_dw_samples, dim3((ncols + 255) / 256, total), dim3(256), 0, st, a, total, ncols, co, A);
+  hipLaunchKernelGGL(k_dw_phases, dim3((nz + 63) / 64, total), dim3(64), 0, st, a, total, (const float*)co, P);
+  const int tz = pick_tz(nz);
+  const dim3 grid((ncols + DG_BC - 1) / DG_BC, (nz + 4 * tz - 1) / (4 * tz));
+  switch (tz) {
+    case 4: hipLaunchKernelGGL(k_dw_product<4>, grid, dim3(64), 0, st, a, total, ncols, (const float2*)P, (const float*)A); break;
+    case 8: hipLaunchKernelGGL(k_dw_product<8>, grid, dim3(128), 0, st, a, total, ncols, (const float2*)P, (const float*)A); break;
+    case 12: hipLaunchKernelGGL(k_dw_product<12>, grid, dim3(192), 0, st, a, total, ncols, (const float2*)P, (const float*)A); break;
+    default: hipLaunchKernelGGL(k_dw_product<16>, grid, dim3(256), 0, st, a, total, ncols, (const float2*)P, (const float*)A); break;
+  }
+  e = hipGetLastError();
+  (void)hipFreeAsync(scr, st);
   if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(e));
   return BPMH_OK;
 }

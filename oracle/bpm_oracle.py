@@ -214,11 +214,28 @@ def intensity_f32(plane):
     return (a * a).astype(F32)
 
 
+_FFT_WORKERS = 1
+
+
+def set_fft_workers(n):
+    """Threads for the 2-D FFTs (1: numpy.fft; more: scipy.fft with that many
+    workers, the same pocketfft transforms).  Only the CPU-baseline timing
+    uses more than one."""
+    global _FFT_WORKERS
+    _FFT_WORKERS = max(1, int(n))
+
+
 def _fft2(a):
+    if _FFT_WORKERS > 1:
+        import scipy.fft
+        return scipy.fft.fft2(a, workers=_FFT_WORKERS)
     return np.fft.fft2(a)
 
 
 def _ifft2(a):
+    if _FFT_WORKERS > 1:
+        import scipy.fft
+        return scipy.fft.ifft2(a, workers=_FFT_WORKERS)
     return np.fft.ifft2(a)
 
 
