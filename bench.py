@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--copy-reference", type=int, default=0, metavar="MIB",
                     help="only run a MIB-sized device copy (counter scale reference)")
+    ap.add_argument("--debye-only", action="store_true",
+                    help="only time the Debye-Wolf workload and print its JSON object")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the z-loop kernel by kernel instead of replaying a HIP graph")
     return ap.parse_args()
@@ -191,6 +193,10 @@ def main():
     a = parse()
     if a.copy_reference:
         copy_reference(a.copy_reference)
+        return
+    if a.debye_only:
+        import torch
+        print(json.dumps({"debye": debye_extra(torch)}), flush=True)
         return
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
