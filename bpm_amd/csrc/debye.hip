@@ -289,6 +289,38 @@ __global__ void k_debye_mirror(int nx0, int ny0, int nz0, const float* uo, const
   if (ez) { float2 v = ezo[so]; if (conj) v.y = -v.y; ez[sf] = v; }
 }
 
+// The same spread, two x-neighbours per lane with 16-byte (8-byte for I)
+// streaming stores: the outputs are written once and not read back here.
+// Needs even nx0 (the caller checks it) and 16-byte aligned Ex/Ey/Ez, 8-byte
+// aligned I.
+typedef float dw_f4v __attribute__((ext_vector_type(4)));
+typedef float dw_f2v __attribute__((ext_vector_type(2)));
+__global__ void __launch_bounds__(128) k_debye_mirror2(int nx0, int ny0, int nz0, const float* uo,
+                                                       const float2* exo, const float2* eyo,
+                                                       const float2* ezo, float* u, float2* ex,
+                                                       float2* ey, float2* ez) {
+  const int fx = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+  const int fy = blockIdx.y;
+  const int fz = blockIdx.z;
+  if (fx >= nx0) return;
+  const int nx = nx0 / 2 + 1, ny = ny0 / 2 + 1;
+  const int ox0 = abs(fx - nx0 / 2), ox1 = abs(fx + 1 - nx0 / 2);
+  const int oy = abs(fy - ny0 / 2), oz = abs(fz - nz0 / 2);
+  const size_t row = ((size_t)oz * ny + oy) * nx;
+  const size_t sf = ((size_t)fz * ny0 + fy) * nx0 + fx;
+  const bool conj = fz <= nz0 / 2;
+  if (u) __builtin_nontemporal_store(dw_f2v{uo[row + ox0], uo[row + ox1]}, (dw_f2v*)(u + sf));
+  const float2* src[3] = {exo, eyo, ezo};
+  float2* dst[3] = {ex, ey, ez};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (!dst[q]) continue;
+    float2 a = src[q][row + ox0], b = src[q][row + ox1];
+    if (conj) { a.y = -a.y; b.y = -b.y; }
+    __builtin_nontemporal_store(dw_f4v{a.x, a.y, b.x, b.y}, (dw_f4v*)(dst[q] + sf));
+  }
+}
+
 __global__ void k_bessel_fill(int n, const float* x, float* out, int count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) out[i] = (float)bpmbessel::jn(n, (double)x[i]);
@@ -392,10 +424,19 @@ extern "C" int bpmh_debye_mirror(int nx0, int ny0, int nz0, const float* u_oct,
   if ((u && !u_oct) || (ex && !ex_oct) || (ey && !ey_oct) || (ez && !ez_oct))
     return bpmh_set_error(BPMH_ERR_INVALID, "output requested without its octant");
   if (ny0 > 65535 || nz0 > 65535) return bpmh_set_error(BPMH_ERR_INVALID, "grid too large");
-  dim3 grid((nx0 + 127) / 128, ny0, nz0);
-  hipLaunchKernelGGL(k_debye_mirror, grid, dim3(128), 0, (hipStream_t)stream, nx0, ny0, nz0,
-                     u_oct, (const float2*)ex_oct, (const float2*)ey_oct, (const float2*)ez_oct,
-                     u, (float2*)ex, (float2*)ey, (float2*)ez);
+  const bool aligned = ((uintptr_t)u % 8 == 0) && ((uintptr_t)ex % 16 == 0) &&
+                       ((uintptr_t)ey % 16 == 0) && ((uintptr_t)ez % 16 == 0);
+  if (aligned) {
+    dim3 grid((nx0 / 2 + 127) / 128, ny0, nz0);
+    hipLaunchKernelGGL(k_debye_mirror2, grid, dim3(128), 0, (hipStream_t)stream, nx0, ny0, nz0,
+                       u_oct, (const float2*)ex_oct, (const float2*)ey_oct, (const float2*)ez_oct,
+                       u, (float2*)ex, (float2*)ey, (float2*)ez);
+  } else {
+    dim3 grid((nx0 + 127) / 128, ny0, nz0);
+    hipLaunchKernelGGL(k_debye_mirror, grid, dim3(128), 0, (hipStream_t)stream, nx0, ny0, nz0,
+                       u_oct, (const float2*)ex_oct, (const float2*)ey_oct, (const float2*)ez_oct,
+                       u, (float2*)ex, (float2*)ey, (float2*)ez);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(e));
   return BPMH_OK;

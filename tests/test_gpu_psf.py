@@ -137,3 +137,30 @@ def test_cylindrical_lens(bpm):
     rp = P.cylindrical(40, 1, -.1 * 39 / 2., .1 * 39 / 2., -.7, -.7, .5 / 1.2, np.arcsin(.5 / 1.2), 90,
                        plane_variant=True)[0]
     assert O.rel_l2(pl, np.repeat(rp[0][:, None], 16, -1)) < TOL
+
+
+def test_mirror_vector_and_unaligned_paths_agree(bpm):
+    """bpmh_debye_mirror writes two x-neighbours per lane with 16-byte stores
+    when the outputs are aligned, and one voxel per lane otherwise; both must
+    equal the oracle's mirror (pure data movement: bit-exact)."""
+    import torch
+    from bpm_amd import _lib
+    say("case mirror aligned / unaligned")
+    rng = np.random.default_rng(7)
+    shape = (12, 10, 8)
+    nx, ny, nz = (s // 2 + 1 for s in shape)
+    u = rng.standard_normal((nz, ny, nx)).astype(np.float32)
+    e = [(rng.standard_normal((nz, ny, nx)) + 1j * rng.standard_normal((nz, ny, nx))).astype(np.complex64)
+         for _ in range(3)]
+    ref = P.mirror_octant(u, *e, shape)
+    dev = [_lib.to_device(a, a.dtype.type) for a in [u] + e]
+    n = shape[0] * shape[1] * shape[2]
+    for offset in (0, 1):  # offset 1 element: 8-byte aligned complex outputs -> scalar path
+        outs = [torch.zeros(n + 1, dtype=torch.float32, device="cuda")] + \
+               [torch.zeros(n + 1, dtype=torch.complex64, device="cuda") for _ in range(3)]
+        views = [o[offset:offset + n] for o in outs]
+        _lib.call("bpmh_debye_mirror", shape[0], shape[1], shape[2], *[_lib.ptr(d) for d in dev],
+                  *[_lib.ptr(v) for v in views], _lib.stream_handle())
+        torch.cuda.synchronize()
+        for got, r in zip(views, ref):
+            assert np.array_equal(_lib.to_numpy(got).reshape(r.shape), r)
