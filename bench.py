@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-debye", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-differentiable", action="store_true")
+    ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--copy-reference", type=int, default=0, metavar="MIB",
                     help="only run a MIB-sized device copy (counter scale reference)")
     ap.add_argument("--debye-only", action="store_true",
@@ -174,6 +176,97 @@ def measured_traffic(kernel, nx, ny, shape):
                 return {"bytes": float(line[len(key):].split()[0]),
                         "source": "profiles/bpm_1024x1024_pass_traffic.md"}
     return None
+
+
+def small_extra(torch, seed):
+    """The 256^2 x 256 plain workload (20 spheres + a centred r = 2 um sphere,
+    plane wave at normal incidence, n0 1.33), Bpm3d path, exit field +
+    summed intensity, graph-replayed; batch 1 and a batch of 16 fields
+    through the same dn."""
+    from bpm_amd import _lib
+    from bpm_amd import bpm3d as B
+    from bpm_amd.workloads import plain_small_workload
+    units, lam, n0 = (.1, .1, .1), .5, 1.33
+    u0, dn = plain_small_workload(seed)
+    nz, ny, nx = dn.shape
+    dn_g = _lib.to_device(dn, np.float32)
+    H = B.propagator_table(nx, ny, units, lam, n0, _lib.H_F64)
+    _lib.check(_lib.load().bpmh_plan_set_graph(B.get_plan(nx, ny), 1))
+    out = {"workload": "256^2 x 256, 21 spheres, plane wave, Bpm3d path, exit field + summed intensity"}
+    for batch in (1, 16):
+        u0_g = _lib.to_device(np.broadcast_to(u0, (batch, ny, nx)), np.complex64)
+        plane = torch.empty_like(u0_g)
+        acc = torch.empty((batch, ny, nx), dtype=torch.float32, device="cuda")
+
+        def step():
+            plane.copy_(u0_g)
+            B.propagate_fields(plane, nz, units, lam, n0, dn=dn_g, dn_kind=_lib.DN_F32,
+                               mean_mode="ignore", H=H, out_sum=acc)
+
+        for _ in range(3):
+            step()
+        torch.cuda.synchronize()
+        reps = 20
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(reps):
+            step()
+        ev1.record()
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / reps
+        vs = nx * ny * (nz - 1) * batch / (ms * 1e-3)
+        out["batch%d" % batch] = {"ms_per_propagation": ms, "voxel_steps_per_s": vs,
+                                  "effective_GBps_44B": 44. * vs / 1e9}
+    return out
+
+
+def differentiable_extra(seed):
+    """The differentiable workload: 512^2 x 256 planes, 16 plane-wave angles,
+    loss = sum |u_exit - target|^2 with targets through a second seeded dn,
+    gradient in dn by the hand-written adjoint with checkpoints every 16
+    planes.  Times forward + loss + backward."""
+    import torch
+    from bpm_amd import adjoint
+    from bpm_amd.spheres import spheres_dn_device
+    from bpm_amd.workloads import random_spheres
+    nz, ny, nx, nb = 256, 512, 512, 16
+    units, lam, n0 = (.1, .1, .1), .5, 1.33
+    rng = np.random.default_rng(seed)
+    size = (nx, ny, nz)
+    dn = spheres_dn_device(size, units, *random_spheres(rng, size, units, 30, (.4, 1.2), (.01, .04)))
+    dn2 = spheres_dn_device(size, units, *random_spheres(rng, size, units, 30, (.4, 1.2), (.01, .04)))
+    th = rng.uniform(-.2, .2, (nb, 2))
+    k = 2 * np.pi * n0 / lam
+    x = np.arange(nx) * units[0]
+    y = np.arange(ny) * units[1]
+    u0 = torch.from_numpy(np.exp(-1j * k * (np.sin(th[:, 0])[:, None, None] * x[None, None, :] +
+                                            np.sin(th[:, 1])[:, None, None] * y[None, :, None]))
+                          .astype(np.complex64)).cuda()
+    target = adjoint.propagate(dn2, u0, units, lam, n0).detach()
+
+    def step():
+        d = dn.clone().requires_grad_(True)
+        out = adjoint.propagate(d, u0, units, lam, n0, checkpoint=16)
+        (out - target).abs().pow(2).sum().backward()
+        return d.grad
+
+    step()
+    torch.cuda.synchronize()
+    reps = 3
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        step()
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    vsteps = nx * ny * (nz - 1) * nb
+    return {"workload": "512^2 x 256 planes, 16 angles, L2 loss vs second-dn targets, "
+                        "gradient in dn, checkpoint every 16 planes",
+            "ms_forward_backward": ms, "forward_voxel_steps": vsteps,
+            "voxel_steps_per_s_forward_equivalent": vsteps / (ms * 1e-3)}
 
 
 def copy_reference(mib):
@@ -318,6 +411,10 @@ def main():
         out["debye"] = debye_extra(torch)
     if not a.no_served:
         out["served"] = served_extra(a.seed)
+    if not a.no_differentiable:
+        out["differentiable"] = differentiable_extra(a.seed)
+    if not a.no_small:
+        out["small_256"] = small_extra(torch, a.seed)
     if not a.no_cpu_baseline:
         from bpm_amd.workloads import spheres_dn
         planes = min(a.cpu_sample_planes, Nz)

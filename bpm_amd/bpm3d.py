@@ -19,19 +19,24 @@ _MEAN = {"ignore": _lib.MEAN_IGNORE, "simple": _lib.MEAN_SIMPLE, "weighted": _li
 _plans = {}
 
 
-def get_plan(Nx, Ny):
-    """Cached libbpmhip plan (FFT twiddles + scratch) for an (Ny, Nx) grid on the current device."""
+def get_plan(Nx, Ny, batch=1):
+    """Cached libbpmhip plan (FFT twiddles + scratch) for an (Ny, Nx) grid on
+    the current device, set to the autotuned launch shapes for ``batch``
+    fields (the cached choice of the nearest tuned batch size); batch None
+    leaves the plan's launch shapes as they are."""
     t = _lib.torch()
     key = (int(Nx), int(Ny), t.cuda.current_device())
     p = _plans.get(key)
     if p is None:
         h = ctypes.c_void_p()
         _lib.call("bpmh_plan_create", int(Nx), int(Ny), ctypes.byref(h))
-        from .tuning import lookup
-        v = lookup(int(Nx), int(Ny), 1)
-        if v is not None:
-            _lib.call("bpmh_plan_set_variant", h, int(v[0]), int(v[1]))
         p = _plans[key] = h
+    if batch is None:
+        return p
+    from .tuning import lookup
+    v = lookup(int(Nx), int(Ny), int(batch))
+    if v is not None:  # two ints on the plan; a replayed graph is keyed by them
+        _lib.call("bpmh_plan_set_variant", p, int(v[0]), int(v[1]))
     return p
 
 
@@ -74,7 +79,7 @@ def propagator_table(Nx, Ny, units, lam, n0, kind=_lib.H_F32):
 
 def propagate_fields(plane, nz, units, lam, n0=1., dn=None, dn_kind=None, dn_batch_stride=0,
                      mean_mode="ignore", H=None, n_eff=None, dn0=None, absorbing_width=0,
-                     out_field=None, out_intens=None, out_sum=None, scat=None):
+                     out_field=None, out_intens=None, out_sum=None, scat=None, tuned=True):
     """Run nz-1 split-step steps on device for a batch of fields (no host copies).
 
     plane: complex64 tensor (B, Ny, Nx) on the GPU; holds u0 on entry and the
@@ -85,6 +90,8 @@ def propagate_fields(plane, nz, units, lam, n0=1., dn=None, dn_kind=None, dn_bat
            arrays (nz) for "simple".
     out_*: preallocated device tensors or None (see include/bpm_hip.h).
     scat:  None or (sw, gw, plain, out) device tensors for the scattering sums.
+    tuned: apply the autotuned launch shapes for this batch size (False keeps
+           the plan's current ones, as the autotuner needs).
     """
     B, Ny, Nx = plane.shape
     dx, dy, dz = units
@@ -95,7 +102,7 @@ def propagate_fields(plane, nz, units, lam, n0=1., dn=None, dn_kind=None, dn_bat
     sw = gw = plain = sout = None
     if scat is not None:
         sw, gw, plain, sout = scat
-    _lib.call("bpmh_propagate", get_plan(Nx, Ny), int(B), int(nz), _lib.ptr(plane), _lib.ptr(dn),
+    _lib.call("bpmh_propagate", get_plan(Nx, Ny, B if tuned else None), int(B), int(nz), _lib.ptr(plane), _lib.ptr(dn),
               int(dn_kind), int(dn_batch_stride), int(mm), _lib.ptr(H), _lib.ptr(n_eff),
               _lib.ptr(dn0), float(np.float32(n0)), float(np.float32(k0)), float(np.float32(dx)),
               float(np.float32(dy)), float(np.float32(dz)), float(np.float32(k0 * dz)),

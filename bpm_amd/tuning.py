@@ -56,7 +56,7 @@ def autotune(nx, ny, batch=1, nz=65, reps=3, units=(.1, .1, .1), lam=.5, n0=1.33
     """Time all 3x3 variants for one grid/batch; returns (best, table)."""
     from . import bpm3d as B
     t = _lib.torch()
-    plan = B.get_plan(nx, ny)
+    plan = B.get_plan(nx, ny, None)
     rng = np.random.default_rng(0)
     dn = t.from_numpy(rng.uniform(0, .05, (nz, ny, nx)).astype(np.float32)).to(_lib.device())
     u0 = t.ones((batch, ny, nx), dtype=t.complex64, device=_lib.device())
@@ -74,7 +74,7 @@ def autotune(nx, ny, batch=1, nz=65, reps=3, units=(.1, .1, .1), lam=.5, n0=1.33
             def run():
                 plane.copy_(u0)
                 B.propagate_fields(plane, nz, units, lam, n0, dn=dn, dn_kind=_lib.DN_F32,
-                                   mean_mode="ignore", H=H, out_sum=acc)
+                                   mean_mode="ignore", H=H, out_sum=acc, tuned=False)
             run()
             t.cuda.synchronize()
             e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
