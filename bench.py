@@ -218,6 +218,31 @@ def small_extra(torch, seed):
         vs = nx * ny * (nz - 1) * batch / (ms * 1e-3)
         out["batch%d" % batch] = {"ms_per_propagation": ms, "voxel_steps_per_s": vs,
                                   "effective_GBps_44B": 44. * vs / 1e9}
+    # bpm_3d's default mean_mode "weighted" (k-space weighted dn0 each step:
+    # three passes per z-step), batch 1, same outputs
+    u0_g = _lib.to_device(u0[None], np.complex64)
+    plane = torch.empty_like(u0_g)
+    acc = torch.empty((1, ny, nx), dtype=torch.float32, device="cuda")
+
+    def wstep():
+        plane.copy_(u0_g)
+        B.propagate_fields(plane, nz, units, lam, n0, dn=dn_g, dn_kind=_lib.DN_F32,
+                           mean_mode="weighted", out_sum=acc)
+
+    for _ in range(3):
+        wstep()
+    torch.cuda.synchronize()
+    reps = 20
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        wstep()
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    out["weighted_batch1"] = {"ms_per_propagation": ms,
+                              "voxel_steps_per_s": nx * ny * (nz - 1) / (ms * 1e-3)}
     return out
 
 
