@@ -140,15 +140,22 @@ def served_extra(seed):
     srv = BpmServer(hbm_budget=48 << 30, slots=4)
     srv.run(jobs[:2], realtime=False)  # warm-up (plans, propagators)
     r = srv.run(jobs, realtime=True)
+    # capacity: the same jobs all queued at once (no arrival gaps), against
+    # the per-request loop over the same jobs
+    cap = srv.run(jobs, realtime=False)
     p = plain_loop(jobs)
     return {"workload": "24 jobs, Poisson 20/s, grids {128,256,512,1024}^2, depth U{64..512}, "
                         "8-64 angles (tilt <= 0.3 rad), spheres as the 256^2 workload; "
                         "latency = arrival -> results on host",
             "p50_ms": r["p50_s"] * 1e3, "p99_ms": r["p99_s"] * 1e3,
             "served_voxel_steps_per_s": r["throughput"],
-            "plain_loop_voxel_steps_per_s": p["throughput"],
+            "served_wall_s": r["wall_s"],
+            "note": "the Poisson run is bounded by the arrivals (the last job arrives "
+                    "about 1.2 s in); capacity = the same jobs all queued at t = 0",
+            "capacity_voxel_steps_per_s": cap["throughput"], "capacity_wall_s": cap["wall_s"],
+            "plain_loop_voxel_steps_per_s": p["throughput"], "plain_wall_s": p["wall_s"],
             "served_over_plain": r["throughput"] / p["throughput"],
-            "served_wall_s": r["wall_s"], "plain_wall_s": p["wall_s"]}
+            "capacity_over_plain": cap["throughput"] / p["throughput"]}
 
 
 def plan_shape(lib, plan, kernel):
