@@ -46,7 +46,8 @@ bpm_3d_split       _bpm_3d_split  bpm/bpm/bpm_3d.py:863-992
                    boundary planes), chunk-local slice-0 dn0 in weighted mode.
 propagate_single   _Bpm3d_OCL._propagate_single  bpm/bpmclass/_bpm3d_ocl.py:66-134
                    pinned: same plane-wave / energy closed forms; equals
-                   bpm_3d(mean_mode="ignore") up to the H variant.
+                   bpm_3d(mean_mode="ignore") up to the H variant; its streamed
+                   return_sum equals the z-sum of the intensity volume.
 fill_dn_coated     fill_dn  bpm/bpm/kernels/bpm_3d_spheres.cl:31-74
                    pinned: brute-force nearest centre over all points.
 """
@@ -452,13 +453,16 @@ def bpm_3d_split(size, units, lam=.5, u0=None, dn=None, n_volumes=1, n0=1.,
 
 def propagate_single(size, units, lam=.5, n0=1., dn=None, u0=None,
                      return_full=True, return_intensity=False,
-                     absorbing_width=0, precision="c64"):
+                     absorbing_width=0, precision="c64", return_sum=False):
     """Restatement of the class path ``_Bpm3d_OCL._propagate_single``
     (bpm/bpmclass/_bpm3d_ocl.py:66-134) with the fp64-host -> complex64 H of
     bpm/bpmclass/bpm3d.py:88-107 (laid out (Ny,Nx)).
 
     ``return_intensity`` (the reference's missing ``fill_with_energy``) is
-    defined as |u|^2 of the returned planes.
+    defined as |u|^2 of the returned planes.  ``return_sum`` (with
+    return_full=False) returns (last plane, sum_z |u_z|^2) accumulated plane
+    by plane, in float32 for "c64" (as the intensity volume summed in z
+    order) and float64 for "f64".
     """
     f64 = precision == "f64"
     Nx, Ny, Nz = size
@@ -484,6 +488,7 @@ def propagate_single(size, units, lam=.5, n0=1., dn=None, u0=None,
     if return_full:
         out = np.empty((Nz, Ny, Nx), (np.float64 if f64 else F32) if return_intensity else cdt)
         out[0] = inten(plane) if return_intensity else plane
+    acc = inten(plane) if return_sum else None
     for i in range(Nz - 1):
         U = _fft2(plane)
         U = (U * H) if f64 else _cmul(U, H)
@@ -492,9 +497,12 @@ def propagate_single(size, units, lam=.5, n0=1., dn=None, u0=None,
             plane = screen(plane, dn_v[i + 1], unit_k, 0.0, absorbing_width, precision)
         if return_full:
             out[i + 1] = inten(plane) if return_intensity else plane
+        if return_sum:
+            acc = acc + inten(plane)
     if return_full:
         return out
-    return inten(plane) if return_intensity else plane
+    last = inten(plane) if return_intensity else plane
+    return (last, acc) if return_sum else last
 
 
 def fill_dn_coated(size, units, points, dn_inner=.0, rad_inner=0., dn_outer=.1, rad_outer=.4):

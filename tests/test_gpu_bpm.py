@@ -325,3 +325,45 @@ def test_launch_variants_agree(bpm, nx, ny):
             assert torch.equal(p, outs[(1, 1, mode)]), (rv, cv, mode)
     finally:
         _lib.check(lib.bpmh_plan_set_variant(plan, 1, 1))
+
+
+@pytest.mark.parametrize("config", ["256x256x256", "1024x1024x1024"])
+def test_baseline_configs_vs_c64_and_f64_oracles(bpm, config):
+    """The plain workloads of the benchmark (Bpm3d path: float64 propagator
+    rounded to complex64, real dn, exit field + sum_z |u_z|^2) in full:
+    256^2 x 256 (21 spheres, plane wave) and 1024^2 x 1024 (400 spheres,
+    focused Gaussian beam), against the complex64 oracle (bar 1e-4) and the
+    float64 oracle (error stated; the oracle runs about a minute here)."""
+    import torch
+    from bpm_amd import _lib
+    from bpm_amd import bpm3d as B
+    from bpm_amd.workloads import plain_large_workload, plain_small_workload
+    say("case config", config)
+    units = (.1, .1, .1)
+    if config == "256x256x256":
+        u0, dn = plain_small_workload(0)
+    else:
+        u0, dn = plain_large_workload(0)
+    nz, ny, nx = dn.shape
+    dn_g = _lib.to_device(dn, np.float32)
+    plane = _lib.to_device(u0[None], np.complex64)
+    acc = torch.empty((1, ny, nx), dtype=torch.float32, device="cuda")
+    H = B.propagator_table(nx, ny, units, LAM, N0, _lib.H_F64)
+    B.propagate_fields(plane, nz, units, LAM, N0, dn=dn_g, dn_kind=_lib.DN_F32,
+                       mean_mode="ignore", H=H, out_sum=acc)
+    got_u, got_s = _lib.to_numpy(plane)[0], _lib.to_numpy(acc)[0]
+    O.set_fft_workers(min(16, len(__import__("os").sched_getaffinity(0))))
+    try:
+        errs = {}
+        for prec in ("c64", "f64"):
+            t0 = __import__("time").perf_counter()
+            ref_u, ref_s = O.propagate_single((nx, ny, nz), units, LAM, N0, dn, u0,
+                                              return_full=False, precision=prec, return_sum=True)
+            errs[prec] = (O.rel_l2(got_u, ref_u), O.rel_l2(got_s, ref_s))
+            say("   %s oracle %.1f s" % (prec, __import__("time").perf_counter() - t0))
+    finally:
+        O.set_fft_workers(1)
+    say("   exit field rel-L2 vs c64 oracle %.2e, vs f64 oracle %.2e" % (errs["c64"][0], errs["f64"][0]))
+    say("   sum |u|^2  rel-L2 vs c64 oracle %.2e, vs f64 oracle %.2e" % (errs["c64"][1], errs["f64"][1]))
+    assert errs["c64"][0] < TOL and errs["c64"][1] < TOL
+    assert errs["f64"][0] < 1e-3 and errs["f64"][1] < 1e-3

@@ -302,3 +302,21 @@ def test_fill_dn_coated_equals_brute_force_nearest_centre():
     edge = (np.abs(d - .5) < 1e-5) | (np.abs(d - .3) < 1e-5)
     assert np.array_equal(got[~edge], ref[~edge].astype(np.float32))
     assert (got > 0).sum() > 50
+
+
+def test_propagate_single_streamed_sum_equals_volume_sum():
+    # return_sum accumulates |u_z|^2 plane by plane; it must equal the z-sum
+    # (in z order) of the return_full intensity volume and leave the exit
+    # plane unchanged
+    rng = np.random.default_rng(4)
+    dn = rng.uniform(0, .05, (9, 32, 48)).astype(np.float32)
+    u0 = np.exp(1j * rng.uniform(0, 3, (32, 48))).astype(np.complex64)
+    for prec in ("c64", "f64"):
+        kw = dict(size=(48, 32, 9), units=(.1, .1, .1), lam=.5, n0=1.33, dn=dn, u0=u0, precision=prec)
+        vol = O.propagate_single(return_full=True, return_intensity=True, **kw)
+        last, acc = O.propagate_single(return_full=False, return_sum=True, **kw)
+        ref = vol[0].copy()
+        for z in range(1, 9):
+            ref = ref + vol[z]
+        assert np.array_equal(acc, ref)
+        assert np.array_equal(last, O.propagate_single(return_full=False, **kw))
