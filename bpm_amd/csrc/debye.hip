@@ -59,7 +59,13 @@ struct Sample {
   float co, a0, a1, a2;
 };
 
-BPMH_DEV Sample make_sample(const DwArgs& a, int i_n, int i_t, float kr) {
+// The parts of a sample that do not depend on the lateral point: cos t,
+// sin t and the quadrature weight w.
+struct SampleParams {
+  float co, si, w;
+};
+
+BPMH_DEV SampleParams sample_params(const DwArgs& a, int i_n, int i_t) {
   const float al1 = a.alphas[2 * i_n], al2 = a.alphas[2 * i_n + 1];
   const float dt = f_div(f_sub(al2, al1), (float)a.S - 1.f);
   const float t = f_add(al1, f_mul((float)i_t, dt));
@@ -78,15 +84,24 @@ BPMH_DEV Sample make_sample(const DwArgs& a, int i_n, int i_t, float kr) {
       rhs = f_mul(rhs, (float)exp((double)f_mul(f_mul(t, t), a.exp_prefac)));
     w = f_mul(w, rhs);
   }
-  const double ks = (double)f_mul(kr, si);
+  return SampleParams{co, si, w};
+}
+
+// amplitudes w (cos t + 1) J0, w sin t J1, w (cos t - 1) J2 at kr sin t
+BPMH_DEV Sample sample_amps(SampleParams p, float kr) {
+  const double ks = (double)f_mul(kr, p.si);
   double j0, j1, j2;
   bpmbessel::j012(ks, j0, j1, j2);
   Sample s;
-  s.co = co;
-  s.a0 = f_mul(f_mul(w, f_add(co, 1.f)), (float)j0);
-  s.a1 = f_mul(f_mul(w, si), (float)j1);
-  s.a2 = f_mul(f_mul(w, f_sub(co, 1.f)), (float)j2);
+  s.co = p.co;
+  s.a0 = f_mul(f_mul(p.w, f_add(p.co, 1.f)), (float)j0);
+  s.a1 = f_mul(f_mul(p.w, p.si), (float)j1);
+  s.a2 = f_mul(f_mul(p.w, f_sub(p.co, 1.f)), (float)j2);
   return s;
+}
+
+BPMH_DEV Sample make_sample(const DwArgs& a, int i_n, int i_t, float kr) {
+  return sample_amps(sample_params(a, i_n, i_t), kr);
 }
 
 // Ex, Ey, Ez, I from the three integrals (psf_debye.cl:88-109)
@@ -119,8 +134,19 @@ BPMH_DEV void combine(const DwArgs& a, float phi, float2 I0, float2 I1, float2 I
 // co[s] = cos t_s and A[n][s][col] = the three real amplitudes of sample s at
 // lateral point col = j*nx + i; the table is zero-padded to tp samples and
 // cp columns (A has row stride cp, plane stride tp*cp).
+// per-sample parameters (cos t, sin t, w) for the table below
+__global__ void __launch_bounds__(64) k_dw_params(DwArgs a, int total, float* par) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= total) return;
+  const int per_ann = a.S + 1;
+  const SampleParams p = sample_params(a, s / per_ann, s % per_ann);
+  par[3 * s] = p.co;
+  par[3 * s + 1] = p.si;
+  par[3 * s + 2] = p.w;
+}
+
 __global__ void __launch_bounds__(256) k_dw_samples(DwArgs a, int total, int ncols, int tp, int cp,
-                                                    float* co, float* A) {
+                                                    const float* __restrict__ par, float* co, float* A) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   const int s = blockIdx.y;
   if (col >= cp) return;
@@ -135,8 +161,7 @@ __global__ void __launch_bounds__(256) k_dw_samples(DwArgs a, int total, int nco
   const float x = axis_coord(i, a.nx, a.x1, a.x2);
   const float y = axis_coord(j, a.ny, a.y1, a.y2);
   const float kr = k_times(a.lam, __fsqrt_rn(f_add(f_mul(x, x), f_mul(y, y))));
-  const int per_ann = a.S + 1;
-  const Sample sm = make_sample(a, s / per_ann, s % per_ann, kr);
+  const Sample sm = sample_amps(SampleParams{par[3 * s], par[3 * s + 1], par[3 * s + 2]}, kr);
   A[o] = sm.a0;
   A[plane + o] = sm.a1;
   A[2 * plane + o] = sm.a2;
@@ -376,18 +401,22 @@ extern "C" int bpmh_debye_wolf(int nx, int ny, int nz, float x1, float x2, float
       total > 65000)
     return bpmh_set_error(BPMH_ERR_INVALID, "grid x samples too large");
   // stream-ordered scratch, zero-padded to whole product tiles: A (3 x tp x
-  // cp), co (tp), P (tp x zp)
+  // cp), co (tp), P (tp x zp), per-sample parameters (3 x total)
   const int tp = (total + 15) / 16 * 16;
   const int nzt = (nz + 15) / 16, zp = nzt * 16;
   const int cp = (ncols + 16 * DM_CT - 1) / (16 * DM_CT) * (16 * DM_CT);
   const size_t nA = (size_t)3 * tp * cp, nP = (size_t)tp * zp;
   void* scr = nullptr;
-  hipError_t e = hipMallocAsync(&scr, nA * sizeof(float) + nP * sizeof(float2) + tp * sizeof(float) + 256, st);
+  hipError_t e = hipMallocAsync(&scr, nA * sizeof(float) + nP * sizeof(float2) + tp * sizeof(float) +
+                                          3 * (size_t)total * sizeof(float) + 256, st);
   if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_NOMEM, "hipMallocAsync failed for the Debye tables");
   float* A = (float*)scr;
   float2* P = (float2*)(((uintptr_t)(A + nA) + 15) & ~(uintptr_t)15);
   float* co = (float*)(P + nP);
-  hipLaunchKernelGGL(k_dw_samples, dim3((cp + 255) / 256, tp), dim3(256), 0, st, a, total, ncols, tp, cp, co, A);
+  float* par = co + tp;
+  hipLaunchKernelGGL(k_dw_params, dim3((total + 63) / 64), dim3(64), 0, st, a, total, par);
+  hipLaunchKernelGGL(k_dw_samples, dim3((cp + 255) / 256, tp), dim3(256), 0, st, a, total, ncols, tp, cp,
+                     (const float*)par, co, A);
   hipLaunchKernelGGL(k_dw_phases, dim3((zp + 63) / 64, tp), dim3(64), 0, st, a, total, tp, zp, (const float*)co, P);
   hipLaunchKernelGGL(k_dw_mfma, dim3(cp / (16 * DM_CT), (nzt + 3) / 4), dim3(256), 0, st, a, ncols, nzt, tp, cp,
                      (const float2*)P, (const float*)A);
