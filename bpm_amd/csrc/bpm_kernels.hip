@@ -13,7 +13,7 @@
 // so each step reads and writes the plane twice and reads dn[z] once.  The
 // default mean_mode "weighted" of bpm_3d needs the k-space reduction of
 // bpm_3d.py:302-303 before H, so its column pass is split in two (A: FFT_y
-// and per-tile partial sums; B: H(n0+dn0) and IFFT_y).
+// and per-column partial sums; B: H(n0+dn0) and IFFT_y).
 #include "common.hpp"
 #include "fft.hpp"
 #include "bpm_hip.h"
@@ -349,7 +349,7 @@ template <int NY, int V, int PHASE>
 __global__ void __launch_bounds__((ColCfg<NY, V>::THREADS))
 k_col(ColArgs a) {
   constexpr int E = ColCfg<NY, V>::E, T = ColCfg<NY, V>::T, COLS = ColCfg<NY, V>::COLS;
-  // per-column lines 17 float2 longer than the padded line: the COLS lines a
+  // per-column lines 9 float2 longer than the padded line: the COLS lines a
   // wavefront touches together then start in different LDS banks
   __shared__ float2 lds[COLS][ColCfg<NY, V>::COL_LINE];
   const int ntiles = a.nx / COLS;
