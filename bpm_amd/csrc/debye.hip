@@ -187,44 +187,45 @@ __global__ void __launch_bounds__(64) k_dw_phases(DwArgs a, int total, int tp, i
 // lane l holds A[l%16][l/16] of the 16x4 left operand, B[l/16][l%16] of the
 // 4x16 right operand, and D[4(l/16)+j][l%16], j < 4, of the 16x16 result).
 // Left operand: Re or Im of P for 16 planes x 4 samples; right operand: A_n
-// for 4 samples x 16 lateral points.  Each wavefront owns 16 planes x
-// DM_CT*16 lateral points (12 accumulators: Re, Im of I0, I1, I2 per column
-// tile) and reads its operands straight from the tables into registers,
-// the next 16 samples in flight while the current ones are multiplied; no
-// LDS and no barriers, so wavefronts run independently.  The four
-// wavefronts of a work-group share their lateral points (their right
-// operands hit in L1).  (Sample-contiguous tables with one 16-byte operand
-// load per row and batch measured 6 % slower.)
+// for 4 samples x 16 lateral points.  Each wavefront owns up to 2 tiles of
+// 16 planes x DM_CT tiles of 16 lateral points (6 accumulators per tile
+// pair: Re, Im of I0, I1, I2) and reads its operands straight from the
+// tables into registers, the next 16 samples in flight while the current
+// ones are multiplied; no LDS and no barriers, so wavefronts run
+// independently.  The four wavefronts of a work-group share their lateral
+// points (their right operands hit in L1).  (Measured alternatives: one
+// plane tile per wavefront 3 % slower; sample-contiguous tables with one
+// 16-byte operand load per row and batch 6 % slower.)
 constexpr int DM_CT = 2;  // 16-column tiles per wavefront
 typedef float dm_v4f __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(256) k_dw_mfma(DwArgs a, int ncols, int nzt, int tp, int cp,
-                                                const float2* __restrict__ P,
-                                                const float* __restrict__ A) {
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+// ZT 16-plane tiles (consecutive, from zt0) x DM_CT 16-column tiles per
+// wavefront; each loaded operand feeds ZT (left) or DM_CT (right) MFMAs.
+template <int ZT>
+BPMH_DEV void dw_mfma_tiles(const DwArgs& a, int ncols, int zp, int tp, int cp, int zt0, int c0, int l,
+                            const float2* __restrict__ P, const float* __restrict__ A) {
   const int lr = l & 15, lk = l >> 4;
-  const int zt = blockIdx.y * 4 + w;
-  if (zt >= nzt) return;  // whole wavefront; the kernel has no barriers
-  const int c0 = blockIdx.x * (16 * DM_CT);
-  const int zp = nzt * 16;
   const size_t plane = (size_t)tp * cp;
   // the tables are zero-padded to whole tiles (tp samples, zp planes, cp
   // columns), so every load below is in range and needs no condition
-  const float2* Pl = P + zt * 16 + lr;
+  const float2* Pl = P + zt0 * 16 + lr;
   const float* Al = A + c0 + lr;
-  dm_v4f acc[DM_CT][6];
+  dm_v4f acc[ZT][DM_CT][6];
 #pragma unroll
-  for (int t = 0; t < DM_CT; ++t)
+  for (int r = 0; r < ZT; ++r)
 #pragma unroll
-    for (int i = 0; i < 6; ++i) acc[t][i] = dm_v4f{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < DM_CT; ++t)
+#pragma unroll
+      for (int i = 0; i < 6; ++i) acc[r][t][i] = dm_v4f{0.f, 0.f, 0.f, 0.f};
 
-  float2 p[4];
+  float2 p[ZT][4];
   float b[DM_CT][3][4];
   auto load = [&](int s0) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const size_t s = (size_t)(s0 + 4 * u + lk);
-      p[u] = Pl[s * zp];
+#pragma unroll
+      for (int r = 0; r < ZT; ++r) p[r][u] = Pl[s * zp + 16 * r];
 #pragma unroll
       for (int t = 0; t < DM_CT; ++t)
 #pragma unroll
@@ -233,11 +234,12 @@ __global__ void __launch_bounds__(256) k_dw_mfma(DwArgs a, int ncols, int nzt, i
   };
   load(0);
   for (int s0 = 0; s0 < tp; s0 += 16) {
-    float2 pc[4];
+    float2 pc[ZT][4];
     float bc[DM_CT][3][4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      pc[u] = p[u];
+#pragma unroll
+      for (int r = 0; r < ZT; ++r) pc[r][u] = p[r][u];
 #pragma unroll
       for (int t = 0; t < DM_CT; ++t)
 #pragma unroll
@@ -247,12 +249,16 @@ __global__ void __launch_bounds__(256) k_dw_mfma(DwArgs a, int ncols, int nzt, i
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int t = 0; t < DM_CT; ++t)
+      for (int r = 0; r < ZT; ++r)
 #pragma unroll
-        for (int n = 0; n < 3; ++n) {
-          acc[t][2 * n] = __builtin_amdgcn_mfma_f32_16x16x4f32(pc[u].x, bc[t][n][u], acc[t][2 * n], 0, 0, 0);
-          acc[t][2 * n + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(pc[u].y, bc[t][n][u], acc[t][2 * n + 1], 0, 0, 0);
-        }
+        for (int t = 0; t < DM_CT; ++t)
+#pragma unroll
+          for (int n = 0; n < 3; ++n) {
+            acc[r][t][2 * n] =
+                __builtin_amdgcn_mfma_f32_16x16x4f32(pc[r][u].x, bc[t][n][u], acc[r][t][2 * n], 0, 0, 0);
+            acc[r][t][2 * n + 1] =
+                __builtin_amdgcn_mfma_f32_16x16x4f32(pc[r][u].y, bc[t][n][u], acc[r][t][2 * n + 1], 0, 0, 0);
+          }
   }
 
   const size_t out_plane = (size_t)a.nx * a.ny;
@@ -265,13 +271,30 @@ __global__ void __launch_bounds__(256) k_dw_mfma(DwArgs a, int ncols, int nzt, i
     const float y = axis_coord(j, a.ny, a.y1, a.y2);
     const float phi = (float)atan2((double)y, (double)x);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int z = zt * 16 + 4 * lk + q;
-      if (z < a.nz)
-        combine(a, phi, make_float2(acc[t][0][q], acc[t][1][q]), make_float2(acc[t][2][q], acc[t][3][q]),
-                make_float2(acc[t][4][q], acc[t][5][q]), (size_t)z * out_plane + c);
-    }
+    for (int r = 0; r < ZT; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int z = (zt0 + r) * 16 + 4 * lk + q;
+        if (z < a.nz)
+          combine(a, phi, make_float2(acc[r][t][0][q], acc[r][t][1][q]),
+                  make_float2(acc[r][t][2][q], acc[r][t][3][q]),
+                  make_float2(acc[r][t][4][q], acc[r][t][5][q]), (size_t)z * out_plane + c);
+      }
   }
+}
+
+// Wavefront w of work-group (x, y) owns the plane-tile pair 4y + w (16-plane
+// tiles 2(4y+w) and 2(4y+w)+1, the second only if it exists) and the
+// lateral points 32x .. 32x+31.
+__global__ void __launch_bounds__(256) k_dw_mfma(DwArgs a, int ncols, int nzt, int tp, int cp,
+                                                const float2* __restrict__ P,
+                                                const float* __restrict__ A) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int zt0 = 2 * (blockIdx.y * 4 + w);
+  if (zt0 >= nzt) return;  // whole wavefront; the kernel has no barriers
+  const int c0 = blockIdx.x * (16 * DM_CT);
+  if (zt0 + 1 < nzt) dw_mfma_tiles<2>(a, ncols, nzt * 16, tp, cp, zt0, c0, l, P, A);
+  else dw_mfma_tiles<1>(a, ncols, nzt * 16, tp, cp, zt0, c0, l, P, A);
 }
 
 __global__ void __launch_bounds__(64) k_debye_points(DwArgs a, int n) {
@@ -418,7 +441,7 @@ extern "C" int bpmh_debye_wolf(int nx, int ny, int nz, float x1, float x2, float
   hipLaunchKernelGGL(k_dw_samples, dim3((cp + 255) / 256, tp), dim3(256), 0, st, a, total, ncols, tp, cp,
                      (const float*)par, co, A);
   hipLaunchKernelGGL(k_dw_phases, dim3((zp + 63) / 64, tp), dim3(64), 0, st, a, total, tp, zp, (const float*)co, P);
-  hipLaunchKernelGGL(k_dw_mfma, dim3(cp / (16 * DM_CT), (nzt + 3) / 4), dim3(256), 0, st, a, ncols, nzt, tp, cp,
+  hipLaunchKernelGGL(k_dw_mfma, dim3(cp / (16 * DM_CT), ((nzt + 1) / 2 + 3) / 4), dim3(256), 0, st, a, ncols, nzt, tp, cp,
                      (const float2*)P, (const float*)A);
   e = hipGetLastError();
   (void)hipFreeAsync(scr, st);
