@@ -73,12 +73,15 @@ template <int N, int E_>
 struct Base {
   static constexpr int E = E_;
   static constexpr int T = N / E;
-  // LDS layouts, from sweeps of the real alternating loop at 1024^2: one
-  // pad slot per 32 entries for row lines and column lines, and column lines
-  // 9 float2 longer than padded, so the 4 columns a wavefront touches start
-  // in different banks (a 17-slot skew with one pad per 16 was 1.3 us per
-  // z-step slower; row pads of one per 4..64 are within 2 %).
-  static constexpr int ROW_SH = 5, COL_SH = 5;
+  // LDS (layouts of fft.hpp): a row line is N float2 in the swizzled
+  // exchange layout (8 more when lines of fewer than 16 threads share a
+  // 16-lane group, so those lines start in different banks).  Column lines
+  // are padded one per 32 and 9 float2 longer, so the 4 columns a wavefront
+  // touches start in different banks; the swizzled layout with the tile's
+  // lines interleaved removes the bank conflicts there too but measured
+  // slower at 128^2-2048^2 except 256^2 in batches.
+  static constexpr int ROW_LINE = N + (T < 16 ? 8 : 0);
+  static constexpr int COL_SH = 5;
   static constexpr int COL_LINE = N + (N >> COL_SH) + 9;  // float2 per column line
   static constexpr int LDS_LINE = COL_LINE * 8;        // bytes per column line
 };
@@ -219,7 +222,7 @@ k_row(RowArgs a) {
   constexpr int E = RowCfg<NX, V>::E, T = RowCfg<NX, V>::T, LINES = RowCfg<NX, V>::LINES;
   // a line of at most 64 threads lies inside one wavefront (T divides 64)
   constexpr bool WAVE = T <= 64;
-  __shared__ float2 lds[LINES][lds_len<NX, RowCfg<NX, V>::ROW_SH>()];
+  __shared__ float2 lds[LINES][RowCfg<NX, V>::ROW_LINE];
   const int line = threadIdx.x / T;
   const int t = threadIdx.x - line * T;
   const int row = blockIdx.x * LINES + line;  // in [0, batch*ny)
@@ -255,7 +258,7 @@ k_row(RowArgs a) {
   }
 
   if (a.inverse) {
-    line_fft<NX, E, true, RowCfg<NX, V>::ROW_SH, WAVE>(v, lds[line], t, tw);
+    line_fft<NX, E, true, SwzLayout, WAVE>(v, lds[line], t, tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = c_scale(v[m], a.inv_n);
   }
@@ -316,7 +319,7 @@ k_row(RowArgs a) {
     }
   }
 
-  if (a.forward) line_fft<NX, E, false, RowCfg<NX, V>::ROW_SH, WAVE>(v, lds[line], t, tw);
+  if (a.forward) line_fft<NX, E, false, SwzLayout, WAVE>(v, lds[line], t, tw);
 #pragma unroll
   for (int m = 0; m < E; ++m) st_stream(p + t + m * T, v[m]);
 }
@@ -352,6 +355,7 @@ k_col(ColArgs a) {
   // per-column lines 9 float2 longer than the padded line: the COLS lines a
   // wavefront touches together then start in different LDS banks
   __shared__ float2 lds[COLS][ColCfg<NY, V>::COL_LINE];
+  using CL = PadLayout<ColCfg<NY, V>::COL_SH>;
   const int ntiles = a.nx / COLS;
   const int bx = blockIdx.x;
   // adjacent tiles share cache lines: keep them on one XCD (blocks are
@@ -382,7 +386,7 @@ k_col(ColArgs a) {
     }
   }
 
-  if constexpr (PHASE != 2) line_fft<NY, E, false, ColCfg<NY, V>::COL_SH>(v, lds[c], t, tw);
+  if constexpr (PHASE != 2) line_fft<NY, E, false, CL>(v, lds[c], t, tw);
 
   if constexpr (PHASE == 1) {
     const size_t s0 = (size_t)b * a.dn_bstride;
@@ -431,7 +435,7 @@ k_col(ColArgs a) {
       if (t == 0) a.spart[((size_t)b * a.nz + a.z) * a.nx + x] = acc;
       __syncthreads();  // lds[c] is the inverse transform's exchange buffer
     }
-    line_fft<NY, E, true, ColCfg<NY, V>::COL_SH>(v, lds[c], t, tw);
+    line_fft<NY, E, true, CL>(v, lds[c], t, tw);
   }
 
 #pragma unroll

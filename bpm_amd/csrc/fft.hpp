@@ -12,8 +12,9 @@
 // wavefront (coalesced when the line is contiguous, and C-wide segments for a
 // tile of C columns).  Stages: one radix-R0 stage (R0 = N / E^S, no
 // twiddles since Ns = 1) followed by S radix-E stages.  Between stages the
-// data crosses threads through a padded LDS line; the last stage is in place
-// in registers.
+// data crosses threads through an LDS line (each stage with its own
+// bank-conflict-free slot map, see Slot); the last stage is in place in
+// registers.
 #pragma once
 #include "common.hpp"
 
@@ -21,13 +22,91 @@ namespace bpmfft {
 
 constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 
-// LDS slot for logical index p: one pad slot per 2^SH entries breaks the
-// bank aliasing of the scattered Stockham writes at small Ns (the callers
-// choose SH by measurement).
+// LDS slot of logical index p during the exchange after a stage of radix
+// 2^RRL and stride 2^NSL, for a line that owns its buffer.  LDS serves a
+// wavefront's 64-bit accesses in groups of 16 (or 32) lanes over 32 (or 64)
+// 4-byte banks, so an access is conflict-free when the group's slots are
+// distinct in their low A = 4 (or A+1) bits.
+//
+// Reads are contiguous (p = t + m*T) and are conflict-free for any map that
+// XORs low bits with higher ones.  The scattered writes p = base + r*2^NSL
+// put the radix digit r at bits [NSL, NSL+RRL) and push the lane bits
+// j_NSL.. up by RRL, so the low A bits repeat across lanes.  The map XORs the
+// displaced lane bits back into those positions.  The holes [NSL, min(NSL+RRL,
+// A)) get bits from max(NSL+RRL, A) on, and bit A gets bit A+RRL when it does
+// not already hold j_A.  Then the low A (and A+1) bits of the written slots
+// are a bijection of the group's lane bits.  Every map is a permutation of
+// [0, N): each output bit is XORed with a strictly higher input bit, so p is
+// recovered from the top bit down.  The map is linear in the bits and its
+// inputs lie above the radix digit, so Slot(base + r*2^NSL) = Slot(base) ^
+// (r << NSL).
+template <int A, int NSL, int RRL>
+struct Slot {
+  static constexpr int NH = NSL < A ? ((A - NSL) < RRL ? (A - NSL) : RRL) : 0;
+  static constexpr int SRC = (NSL + RRL) > A ? (NSL + RRL) : A;
+  static constexpr bool BITA = A >= 0 && A >= NSL;
+  // the map leaves the radix digit's bits alone: r << NSL can be added
+  static constexpr bool DIGIT_CLEAN = NH == 0 && !(BITA && A < NSL + RRL);
+  __host__ __device__ static constexpr int map(int p) {
+    int f = 0;
+    if (NH > 0) f |= ((p >> SRC) & ((1 << NH) - 1)) << NSL;
+    if (BITA) f |= ((p >> (A + RRL)) & 1) << A;
+    return p ^ f;
+  }
+  // map permutes [0, N) (evaluated at compile time for every stage used)
+  template <int N>
+  static constexpr bool permutes() {
+    bool seen[N] = {};
+    for (int p = 0; p < N; ++p) {
+      const int q = map(p);
+      if (q < 0 || q >= N || seen[q]) return false;
+      seen[q] = true;
+    }
+    return true;
+  }
+};
+
+// Exchange layouts.  wr: LDS index of element base + r*2^NSL written after a
+// stage of radix 2^RRL and stride 2^NSL (base has zero radix-digit bits); rd:
+// LDS index of element t + m*T read back.
+//
+// SwzLayout: the Slot maps above (no padding).
+struct SwzLayout {
+  static constexpr int A = 4;
+  template <int N, int NSL, int RRL>
+  static constexpr bool valid() { return Slot<A, NSL, RRL>::template permutes<N>(); }
+  template <int NSL, int RRL>
+  BPMH_DEV static int wr(int base, int r) {
+    using S = Slot<A, NSL, RRL>;
+    const int x = S::map(base);
+    return S::DIGIT_CLEAN ? x + (r << NSL) : x ^ (r << NSL);
+  }
+  // t < T and m*T share no bits, and the map is linear in the bits, so
+  // Slot(t + m*T) = Slot(t) ^ Slot(m*T); a constant above T is added
+  template <int NSL, int RRL, int T>
+  BPMH_DEV static int rd(int t, int m) {
+    using S = Slot<A, NSL, RRL>;
+    const int y = S::map(t), km = S::map(m * T);
+    return (km & (T - 1)) == 0 ? y + km : y ^ km;
+  }
+};
+
+// PadLayout: one pad slot after every 2^SH elements.
 template <int SH>
-BPMH_DEV int lpad(int p) { return p + (p >> SH); }
-template <int N, int SH>
-constexpr int lds_len() { return N + (N >> SH); }
+struct PadLayout {
+  template <int N, int NSL, int RRL>
+  static constexpr bool valid() { return true; }
+  template <int NSL, int RRL>
+  BPMH_DEV static int wr(int base, int r) {
+    const int p = base + (r << NSL);
+    return p + (p >> SH);
+  }
+  template <int NSL, int RRL, int T>
+  BPMH_DEV static int rd(int t, int m) {
+    const int p = t + m * T;
+    return p + (p >> SH);
+  }
+};
 
 // s * 2^-1/2 with a two-term constant: the float32 rounding of 2^-1/2 alone
 // biases every W8^odd product by -1.7e-8, which compounds over thousands of
@@ -97,12 +176,14 @@ BPMH_DEV void line_sync() {
 
 // One Stockham stage.  w[r-1] = exp(-2 pi i r k/(NS R)), k = t mod NS, the
 // thread's twiddles of this stage (only radix-E stages, B = 1, have NS > 1).
-template <int N, int E, int R, int NS, bool INV, bool LAST, int SH, bool WAVE>
+template <int N, int E, int R, int NS, bool INV, bool LAST, class L, bool WAVE>
 BPMH_DEV void stage(float2 (&v)[E], float2* __restrict__ lds, int t,
                     const float2* __restrict__ w) {
   constexpr int T = N / E;
   constexpr int B = E / R;
+  constexpr int NSL = ilog2c(NS), RRL = ilog2c(R);
   static_assert(NS == 1 || B == 1, "twiddled stages are full radix-E stages");
+  static_assert(L::template valid<N, NSL, RRL>(), "exchange layout must permute the line");
 #pragma unroll
   for (int q = 0; q < B; ++q) {
     const int j = t + q * T;
@@ -121,26 +202,26 @@ BPMH_DEV void stage(float2 (&v)[E], float2* __restrict__ lds, int t,
     } else {
       const int base = (j - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[lpad<SH>(base + r * NS)] = a[r];
+      for (int r = 0; r < R; ++r) lds[L::template wr<NSL, RRL>(base, r)] = a[r];
     }
   }
   if constexpr (!LAST) {
     line_sync<WAVE>();
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = lds[lpad<SH>(t + m * T)];
+    for (int m = 0; m < E; ++m) v[m] = lds[L::template rd<NSL, RRL, T>(t, m)];
     line_sync<WAVE>();
   }
 }
 
 // Radix-E stages from stride NS on; w points at this stage's E-1 twiddles
 // (none for NS == 1).
-template <int N, int E, int NS, bool INV, int SH, bool WAVE>
+template <int N, int E, int NS, bool INV, class L, bool WAVE>
 struct RadixEStages {
   BPMH_DEV static void run(float2 (&v)[E], float2* lds, int t, const float2* w) {
     constexpr bool LAST = (NS * E == N);
-    stage<N, E, E, NS, INV, LAST, SH, WAVE>(v, lds, t, w);
+    stage<N, E, E, NS, INV, LAST, L, WAVE>(v, lds, t, w);
     if constexpr (!LAST)
-      RadixEStages<N, E, NS * E, INV, SH, WAVE>::run(v, lds, t, NS > 1 ? w + (E - 1) : w);
+      RadixEStages<N, E, NS * E, INV, L, WAVE>::run(v, lds, t, NS > 1 ? w + (E - 1) : w);
   }
 };
 
@@ -187,18 +268,20 @@ struct Twiddles {
 
 // Unnormalised DFT (INV=false: exp(-2 pi i jk/N); INV=true: exp(+2 pi i jk/N))
 // of the line held as v[m] = x[t + m*T]; result in the same layout.
-// Every thread of the work-group must call it (it synchronises the group);
-// WAVE = true when the line's T threads form one wavefront (or part of one)
-// and no other thread touches its LDS line.
-template <int N, int E, bool INV, int SH, bool WAVE = false>
+// lds: the line's exchange buffer in layout L (SwzLayout or PadLayout; the
+// pointer carries the line's offset).  Every thread of the work-group must
+// call it (it synchronises the group); WAVE = true when the line's T threads
+// form one wavefront (or part of one) and no other thread touches its LDS
+// line.
+template <int N, int E, bool INV, class L, bool WAVE = false>
 BPMH_DEV void line_fft(float2 (&v)[E], float2* __restrict__ lds, int t,
                        const Twiddles<N, E>& tw) {
   using Sh = Shape<N, E>;
   if constexpr (Sh::R0 > 1) {
-    stage<N, E, Sh::R0, 1, INV, false, SH, WAVE>(v, lds, t, nullptr);
-    RadixEStages<N, E, Sh::R0, INV, SH, WAVE>::run(v, lds, t, tw.w);
+    stage<N, E, Sh::R0, 1, INV, false, L, WAVE>(v, lds, t, nullptr);
+    RadixEStages<N, E, Sh::R0, INV, L, WAVE>::run(v, lds, t, tw.w);
   } else {
-    RadixEStages<N, E, 1, INV, SH, WAVE>::run(v, lds, t, tw.w);
+    RadixEStages<N, E, 1, INV, L, WAVE>::run(v, lds, t, tw.w);
   }
 }
 
