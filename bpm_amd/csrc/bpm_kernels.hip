@@ -608,8 +608,12 @@ extern "C" int bpmh_plan_create(int nx, int ny, bpmh_plan* out) {
   std::vector<float2> tx, ty;
   make_stage_twiddles(nx, row_e(nx), tx);
   make_stage_twiddles(ny, col_e(ny), ty);
+  p->tw_x = nullptr;
+  p->tw_y = nullptr;
   if (hipMalloc(&p->tw_x, tx.size() * sizeof(float2)) != hipSuccess ||
       hipMalloc(&p->tw_y, ty.size() * sizeof(float2)) != hipSuccess) {
+    (void)hipFree(p->tw_x);
+    (void)hipFree(p->tw_y);
     delete p;
     return bpmh_set_error(BPMH_ERR_NOMEM, "hipMalloc failed for twiddles");
   }
@@ -808,6 +812,7 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, 
   if (dn_kind < BPMH_DN_NONE || dn_kind > BPMH_DN_C64)
     return bpmh_set_error(BPMH_ERR_INVALID, "unknown dn kind");
   if (dn_kind != BPMH_DN_NONE && !dn) return bpmh_set_error(BPMH_ERR_INVALID, "null dn");
+  if (dn_batch_stride < 0) return bpmh_set_error(BPMH_ERR_INVALID, "dn batch stride must be >= 0");
   if (mean_mode < BPMH_MEAN_IGNORE || mean_mode > BPMH_MEAN_WEIGHTED)
     return bpmh_set_error(BPMH_ERR_INVALID, "unknown mean mode");
   if (mean_mode == BPMH_MEAN_IGNORE && !H)
@@ -818,7 +823,9 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, 
   if (out_scat && (!scat_w || !g_w || !plain))
     return bpmh_set_error(BPMH_ERR_INVALID, "scattering output needs weights and plane-wave array");
   const int nx = plan->nx, ny = plan->ny;
-  if ((long long)batch * ny > (1LL << 30)) return bpmh_set_error(BPMH_ERR_INVALID, "batch too large");
+  // the column grid's y extent is the batch (at most 65535)
+  if (batch > 65535 || (long long)batch * ny > (1LL << 30))
+    return bpmh_set_error(BPMH_ERR_INVALID, "batch too large");
   hipStream_t s = (hipStream_t)stream;
   const row_fn row = pick_row(nx, plan->row_v);
   const col_fn col0 = pick_col<0>(ny, plan->col_v);
@@ -994,6 +1001,8 @@ extern "C" int bpmh_adjoint_segment(bpmh_plan plan, int batch, int z_lo, int z_h
   if (!plan || !g || !u_seg || !dn || !H || !grad_dn || !gitem)
     return bpmh_set_error(BPMH_ERR_INVALID, "null argument");
   if (batch < 1 || z_lo < 0 || z_hi < z_lo) return bpmh_set_error(BPMH_ERR_INVALID, "bad range");
+  if (batch > 65535 || (long long)batch * plan->ny > (1LL << 30))
+    return bpmh_set_error(BPMH_ERR_INVALID, "batch too large");
   if (absorb < 0) return bpmh_set_error(BPMH_ERR_INVALID, "absorbing width must be >= 0");
   if (z_hi == z_lo) return BPMH_OK;
   const int nx = plan->nx, ny = plan->ny;

@@ -419,10 +419,13 @@ extern "C" int bpmh_debye_wolf(int nx, int ny, int nz, float x1, float x2, float
   a.ex = (float2*)ex; a.ey = (float2*)ey; a.ez = (float2*)ez; a.I = I;
   hipStream_t st = (hipStream_t)stream;
   const int ncols = nx * ny;
-  const int total = (int_steps + 1) * (n_alphas / 2);
-  if ((long long)ncols * (total + 64) > (1LL << 31) || (long long)(nz + 16) * (total + 64) > (1LL << 31) ||
-      total > 65000)
+  // samples index the y extent of two grids (at most 65535); the product
+  // grid's y extent is about nz / 128
+  const long long total_ll = ((long long)int_steps + 1) * (n_alphas / 2);
+  if (total_ll > 65000 || nz > (1 << 23) || (long long)ncols * (total_ll + 64) > (1LL << 31) ||
+      (long long)(nz + 16) * (total_ll + 64) > (1LL << 31))
     return bpmh_set_error(BPMH_ERR_INVALID, "grid x samples too large");
+  const int total = (int)total_ll;
   // stream-ordered scratch, zero-padded to whole product tiles: A (3 x tp x
   // cp), co (tp), P (tp x zp), per-sample parameters (3 x total)
   const int tp = (total + 15) / 16 * 16;

@@ -56,6 +56,15 @@ def test_argument_validation_paths(lib):
     rc = lib.bpmh_debye_wolf(8, 8, 8, 0, 1, 0, 1, 0, 1, .5, al.ctypes.data, 3, 100, 1., 0., 0,
                              0., 0, None, None, None, None, None)
     assert rc == _lib.ERR_INVALID
+    # sample counts beyond the launch grids' y extent, including an
+    # int_steps whose (int_steps + 1) * pairs overflows 32 bits, and too many
+    # planes for the product grid are rejected before any allocation
+    al = np.array([0., .5], np.float32)
+    for steps, nz in ((65000, 8), (2 ** 31 - 1, 8), (100, 1 << 24)):
+        rc = lib.bpmh_debye_wolf(8, 8, nz, 0, 1, 0, 1, 0, 1, .5, al.ctypes.data, 2, steps, 1.,
+                                 0., 0, 0., 0, None, None, None, None, None)
+        assert rc == _lib.ERR_INVALID, (steps, nz)
+        assert b"too large" in lib.bpmh_last_error()
     rc = lib.bpmh_debye_mirror(9, 8, 8, None, None, None, None, None, None, None, None, None)
     assert rc == _lib.ERR_UNSUPPORTED
     rc = lib.bpmh_bessel_fill(3, None, None, 4, None)
