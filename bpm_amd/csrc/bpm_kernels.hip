@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace bpmfft;
@@ -75,11 +76,11 @@ struct Base {
   static constexpr int T = N / E;
   // LDS (layouts of fft.hpp): a row line is N float2 in the swizzled
   // exchange layout (8 more when lines of fewer than 16 threads share a
-  // 16-lane group, so those lines start in different banks).  Column lines
-  // are padded one per 32 and 9 float2 longer, so the 4 columns a wavefront
-  // touches start in different banks; the swizzled layout with the tile's
-  // lines interleaved removes the bank conflicts there too but measured
-  // slower at 128^2-2048^2 except 256^2 in batches.
+  // 16-lane group, so those lines start in different banks).  Column tiles
+  // of 4 or more columns use the swizzled layout for C = COLS lines per
+  // 16-lane group, lines N + 16/COLS apart; narrower tiles keep lines padded
+  // one per 32 and 9 float2 longer (the swizzle measured slower there).
+  // COL_LINE, the padded length, bounds both.
   static constexpr int ROW_LINE = N + (T < 16 ? 8 : 0);
   static constexpr int COL_SH = 5;
   static constexpr int COL_LINE = N + (N >> COL_SH) + 9;  // float2 per column line
@@ -258,7 +259,7 @@ k_row(RowArgs a) {
   }
 
   if (a.inverse) {
-    line_fft<NX, E, true, SwzLayout, WAVE>(v, lds[line], t, tw);
+    line_fft<NX, E, true, SwzLayout<4>, WAVE>(v, lds[line], t, tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = c_scale(v[m], a.inv_n);
   }
@@ -319,7 +320,7 @@ k_row(RowArgs a) {
     }
   }
 
-  if (a.forward) line_fft<NX, E, false, SwzLayout, WAVE>(v, lds[line], t, tw);
+  if (a.forward) line_fft<NX, E, false, SwzLayout<4>, WAVE>(v, lds[line], t, tw);
 #pragma unroll
   for (int m = 0; m < E; ++m) st_stream(p + t + m * T, v[m]);
 }
@@ -352,10 +353,14 @@ template <int NY, int V, int PHASE>
 __global__ void __launch_bounds__((ColCfg<NY, V>::THREADS))
 k_col(ColArgs a) {
   constexpr int E = ColCfg<NY, V>::E, T = ColCfg<NY, V>::T, COLS = ColCfg<NY, V>::COLS;
-  // per-column lines 9 float2 longer than the padded line: the COLS lines a
-  // wavefront touches together then start in different LDS banks
-  __shared__ float2 lds[COLS][ColCfg<NY, V>::COL_LINE];
-  using CL = PadLayout<ColCfg<NY, V>::COL_SH>;
+  // swizzled lines 16/COLS float2 apart, or padded lines 9 float2 longer
+  // than padded: either way the COLS lines a 16-lane group touches together
+  // start in different LDS banks
+  constexpr bool SWZ = COLS >= 4;
+  static_assert(COLS <= 16, "at most 16 columns per tile");
+  constexpr int LINE = SWZ ? NY + 16 / COLS : ColCfg<NY, V>::COL_LINE;
+  __shared__ float2 lds[COLS][LINE];
+  using CL = std::conditional_t<SWZ, SwzLayout<4 - ilog2c(COLS)>, PadLayout<ColCfg<NY, V>::COL_SH>>;
   const int ntiles = a.nx / COLS;
   const int bx = blockIdx.x;
   // adjacent tiles share cache lines: keep them on one XCD (blocks are

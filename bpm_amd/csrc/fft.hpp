@@ -23,10 +23,14 @@ namespace bpmfft {
 constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 
 // LDS slot of logical index p during the exchange after a stage of radix
-// 2^RRL and stride 2^NSL, for a line that owns its buffer.  LDS serves a
-// wavefront's 64-bit accesses in groups of 16 (or 32) lanes over 32 (or 64)
-// 4-byte banks, so an access is conflict-free when the group's slots are
-// distinct in their low A = 4 (or A+1) bits.
+// 2^RRL and stride 2^NSL.  LDS serves a wavefront's 64-bit accesses in groups
+// of 16 lanes over 32 4-byte banks (the bank-conflict counters confirm this
+// model on gfx950), so an access is conflict-free when the group's slots are
+// distinct modulo 16.  A line whose threads fill a group needs its slots
+// distinct in their low A = 4 bits; C lines sharing a group (column tiles:
+// lane c of every C works on line c) each need A = 4 - log2(C) bits, with the
+// line buffers offset by 16/C slots.  The map also makes the low A+1 bits
+// distinct, for a 32-lane service group.
 //
 // Reads are contiguous (p = t + m*T) and are conflict-free for any map that
 // XORs low bits with higher ones.  The scattered writes p = base + r*2^NSL
@@ -70,9 +74,11 @@ struct Slot {
 // stage of radix 2^RRL and stride 2^NSL (base has zero radix-digit bits); rd:
 // LDS index of element t + m*T read back.
 //
-// SwzLayout: the Slot maps above (no padding).
+// SwzLayout<A>: the Slot maps above (no padding); A = 4 for a line whose
+// threads fill 16-lane groups, A = 4 - log2(C) when C lines share each group.
+template <int A_ = 4>
 struct SwzLayout {
-  static constexpr int A = 4;
+  static constexpr int A = A_;
   template <int N, int NSL, int RRL>
   static constexpr bool valid() { return Slot<A, NSL, RRL>::template permutes<N>(); }
   template <int NSL, int RRL>
