@@ -70,36 +70,45 @@ struct Slot {
   }
 };
 
-// Exchange layouts.  wr: LDS index of element base + r*2^NSL written after a
-// stage of radix 2^RRL and stride 2^NSL (base has zero radix-digit bits); rd:
-// LDS index of element t + m*T read back.
+// Exchange layouts.  wr: offset in the line's LDS buffer of element
+// base + r*2^NSL written after a stage of radix 2^RRL and stride 2^NSL (base
+// has zero radix-digit bits); rd: offset of element t + m*T read back.
+// Offsets are bytes when BYTES (a per-element XOR then needs no shift after
+// it), else element indices (which let the compiler fold constant parts into
+// the LDS instructions' offsets); each layout uses the form that compiles to
+// fewer instructions for the kernels here.
 //
 // SwzLayout<A>: the Slot maps above (no padding); A = 4 for a line whose
-// threads fill 16-lane groups, A = 4 - log2(C) when C lines share each group.
+// threads fill 16-lane groups (bytes), A = 4 - log2(C) when C lines share
+// each group (element indices).
 template <int A_ = 4>
 struct SwzLayout {
   static constexpr int A = A_;
+  static constexpr bool BYTES = A == 4;
+  static constexpr int SH = BYTES ? 3 : 0;
   template <int N, int NSL, int RRL>
   static constexpr bool valid() { return Slot<A, NSL, RRL>::template permutes<N>(); }
   template <int NSL, int RRL>
   BPMH_DEV static int wr(int base, int r) {
     using S = Slot<A, NSL, RRL>;
-    const int x = S::map(base);
-    return S::DIGIT_CLEAN ? x + (r << NSL) : x ^ (r << NSL);
+    const int x = S::map(base) << SH;
+    const int d = r << (NSL + SH);
+    return S::DIGIT_CLEAN ? x + d : x ^ d;
   }
   // t < T and m*T share no bits, and the map is linear in the bits, so
   // Slot(t + m*T) = Slot(t) ^ Slot(m*T); a constant above T is added
   template <int NSL, int RRL, int T>
   BPMH_DEV static int rd(int t, int m) {
     using S = Slot<A, NSL, RRL>;
-    const int y = S::map(t), km = S::map(m * T);
-    return (km & (T - 1)) == 0 ? y + km : y ^ km;
+    const int y = S::map(t) << SH, km = S::map(m * T);
+    return (km & (T - 1)) == 0 ? y + (km << SH) : y ^ (km << SH);
   }
 };
 
 // PadLayout: one pad slot after every 2^SH elements.
 template <int SH>
 struct PadLayout {
+  static constexpr bool BYTES = false;
   template <int N, int NSL, int RRL>
   static constexpr bool valid() { return true; }
   template <int NSL, int RRL>
@@ -113,6 +122,12 @@ struct PadLayout {
     return p + (p >> SH);
   }
 };
+
+template <class L>
+BPMH_DEV float2& lds_at(float2* lds, int off) {
+  if constexpr (L::BYTES) return *reinterpret_cast<float2*>(reinterpret_cast<char*>(lds) + off);
+  else return lds[off];
+}
 
 // s * 2^-1/2 with a two-term constant: the float32 rounding of 2^-1/2 alone
 // biases every W8^odd product by -1.7e-8, which compounds over thousands of
@@ -208,13 +223,13 @@ BPMH_DEV void stage(float2 (&v)[E], float2* __restrict__ lds, int t,
     } else {
       const int base = (j - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[L::template wr<NSL, RRL>(base, r)] = a[r];
+      for (int r = 0; r < R; ++r) lds_at<L>(lds, L::template wr<NSL, RRL>(base, r)) = a[r];
     }
   }
   if constexpr (!LAST) {
     line_sync<WAVE>();
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = lds[L::template rd<NSL, RRL, T>(t, m)];
+    for (int m = 0; m < E; ++m) v[m] = lds_at<L>(lds, L::template rd<NSL, RRL, T>(t, m));
     line_sync<WAVE>();
   }
 }
