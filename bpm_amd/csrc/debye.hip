@@ -21,7 +21,9 @@
 #include "bpm_hip.h"
 
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <mutex>
 
 int bpmh_set_error(int code, const char* msg);
 
@@ -404,6 +406,42 @@ static int fill_args(DwArgs& a, float lam, const float* alphas, int n_alphas, in
   return BPMH_OK;
 }
 
+// The library's stream-ordered memory pool for the per-call Debye tables, one
+// per device, created on first use.  It keeps freed memory instead of
+// returning it to the driver, so repeated calls reuse the tables' pages
+// (with the HIP default of releasing at every synchronisation, re-mapping
+// ~110 MB per 256^3 call cost 0.47 -> 0.42 ms).  It holds up to the largest
+// table set used until the process ends.
+static hipError_t table_pool(hipMemPool_t* out) {
+  static std::mutex mu;
+  static hipMemPool_t pools[64] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!pools[dev]) {
+    hipMemPoolProps props;
+    memset(&props, 0, sizeof props);
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    hipMemPool_t pool = nullptr;
+    e = hipMemPoolCreate(&pool, &props);
+    if (e != hipSuccess) return e;
+    uint64_t keep = UINT64_MAX;
+    e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    if (e != hipSuccess) {
+      (void)hipMemPoolDestroy(pool);
+      return e;
+    }
+    pools[dev] = pool;
+  }
+  *out = pools[dev];
+  return hipSuccess;
+}
+
 extern "C" int bpmh_debye_wolf(int nx, int ny, int nz, float x1, float x2, float y1, float y2,
                                float z1, float z2, float lam, const float* alphas, int n_alphas,
                                int int_steps, float ex0, float ey0, int variant, float sig,
@@ -433,9 +471,13 @@ extern "C" int bpmh_debye_wolf(int nx, int ny, int nz, float x1, float x2, float
   const int cp = (ncols + 16 * DM_CT - 1) / (16 * DM_CT) * (16 * DM_CT);
   const size_t nA = (size_t)3 * tp * cp, nP = (size_t)tp * zp;
   void* scr = nullptr;
-  hipError_t e = hipMallocAsync(&scr, nA * sizeof(float) + nP * sizeof(float2) + tp * sizeof(float) +
-                                          3 * (size_t)total * sizeof(float) + 256, st);
-  if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_NOMEM, "hipMallocAsync failed for the Debye tables");
+  hipMemPool_t pool = nullptr;
+  hipError_t e = table_pool(&pool);
+  if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(e));
+  e = hipMallocFromPoolAsync(&scr, nA * sizeof(float) + nP * sizeof(float2) + tp * sizeof(float) +
+                                       3 * (size_t)total * sizeof(float) + 256,
+                             pool, st);
+  if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_NOMEM, "stream-ordered allocation failed for the Debye tables");
   float* A = (float*)scr;
   float2* P = (float2*)(((uintptr_t)(A + nA) + 15) & ~(uintptr_t)15);
   float* co = (float*)(P + nP);
