@@ -170,6 +170,48 @@ __global__ void __launch_bounds__(256) k_dw_samples(DwArgs a, int total, int nco
   if (col == 0) co[s] = sm.co;
 }
 
+// The same table when the x and y axes are identical (nx == ny, x1 == y1,
+// x2 == y2, as for the octant of a centred grid): kr(i, j) and kr(j, i) are
+// then bit-identical (the same coordinates, and x^2 + y^2 rounds the same in
+// either order), so one thread evaluates the Bessel terms for the pair i <= j
+// and writes both columns.  Threads past the npairs pairs zero the padding
+// columns [ncols, cp); rows s >= total are zero.
+__global__ void __launch_bounds__(256) k_dw_samples_sym(DwArgs a, int total, int ncols, int tp, int cp,
+                                                        int npairs, const float* __restrict__ par,
+                                                        float* co, float* A) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  const int npad = cp - ncols;
+  if (idx >= npairs + npad) return;
+  const size_t plane = (size_t)tp * cp;
+  float* row = A + (size_t)s * cp;
+  if (idx >= npairs) {  // padding column
+    const int col = ncols + (idx - npairs);
+    row[col] = row[plane + col] = row[2 * plane + col] = 0.f;
+    return;
+  }
+  // pair idx -> (i, j), i <= j, in the order (0,0), (0,1), (1,1), (0,2), ...
+  int j = (int)((sqrtf(8.f * (float)idx + 1.f) - 1.f) * 0.5f);
+  while (j * (j + 1) / 2 > idx) --j;
+  while ((j + 1) * (j + 2) / 2 <= idx) ++j;
+  const int i = idx - j * (j + 1) / 2;
+  const int c1 = j * a.nx + i, c2 = i * a.nx + j;
+  if (s >= total) {
+    row[c1] = row[plane + c1] = row[2 * plane + c1] = 0.f;
+    row[c2] = row[plane + c2] = row[2 * plane + c2] = 0.f;
+    if (idx == 0) co[s] = 0.f;
+    return;
+  }
+  const float x = axis_coord(i, a.nx, a.x1, a.x2);
+  const float y = axis_coord(j, a.ny, a.y1, a.y2);
+  const float kr = k_times(a.lam, __fsqrt_rn(f_add(f_mul(x, x), f_mul(y, y))));
+  const Sample sm = sample_amps(SampleParams{par[3 * s], par[3 * s + 1], par[3 * s + 2]}, kr);
+  row[c1] = row[c2] = sm.a0;
+  row[plane + c1] = row[plane + c2] = sm.a1;
+  row[2 * plane + c1] = row[2 * plane + c2] = sm.a2;
+  if (idx == 0) co[s] = sm.co;
+}
+
 // P[s][z] = e^{i kz cos t_s}, with the float32 argument kz*cos t; zero-padded
 // to tp samples and zp planes (row stride zp)
 __global__ void __launch_bounds__(64) k_dw_phases(DwArgs a, int total, int tp, int zp, const float* co,
@@ -483,8 +525,15 @@ extern "C" int bpmh_debye_wolf(int nx, int ny, int nz, float x1, float x2, float
   float* co = (float*)(P + nP);
   float* par = co + tp;
   hipLaunchKernelGGL(k_dw_params, dim3((total + 63) / 64), dim3(64), 0, st, a, total, par);
-  hipLaunchKernelGGL(k_dw_samples, dim3((cp + 255) / 256, tp), dim3(256), 0, st, a, total, ncols, tp, cp,
-                     (const float*)par, co, A);
+  const bool sym = nx == ny && x1 == y1 && x2 == y2;
+  if (sym) {
+    const int npairs = nx * (nx + 1) / 2;
+    hipLaunchKernelGGL(k_dw_samples_sym, dim3((npairs + (cp - ncols) + 255) / 256, tp), dim3(256), 0, st, a,
+                       total, ncols, tp, cp, npairs, (const float*)par, co, A);
+  } else {
+    hipLaunchKernelGGL(k_dw_samples, dim3((cp + 255) / 256, tp), dim3(256), 0, st, a, total, ncols, tp, cp,
+                       (const float*)par, co, A);
+  }
   hipLaunchKernelGGL(k_dw_phases, dim3((zp + 63) / 64, tp), dim3(64), 0, st, a, total, tp, zp, (const float*)co, P);
   hipLaunchKernelGGL(k_dw_mfma, dim3(cp / (16 * DM_CT), ((nzt + 1) / 2 + 3) / 4), dim3(256), 0, st, a, ncols, nzt, tp, cp,
                      (const float2*)P, (const float*)A);

@@ -164,3 +164,26 @@ def test_mirror_vector_and_unaligned_paths_agree(bpm):
         torch.cuda.synchronize()
         for got, r in zip(views, ref):
             assert np.array_equal(_lib.to_numpy(got).reshape(r.shape), r)
+
+
+def test_debye_symmetric_sample_table_is_bit_identical(bpm):
+    """With identical x and y axes bpmh_debye_wolf fills its sample table once
+    per pair of columns (i, j), i <= j.  An (n, n+1) grid whose first n y
+    coordinates equal the x coordinates exactly (power-of-two spacing) runs
+    the general kernel; its first n rows must equal the symmetric run bit for
+    bit, and the oracle to the usual tolerance."""
+    from bpm_amd import focus_fields as Q
+    from bpm_amd import _lib
+    n, nz, d, lam = 17, 6, 0.0625, .5 / 1.33
+    alphas = Q._alphas(1.1, 1.33)
+    say("case Debye symmetric table, n", n)
+    sym = Q._grid_call(n, n, nz, 0., (n - 1) * d, 0., (n - 1) * d, 0., (nz - 1) * d, lam, alphas,
+                       64, _lib.DW_INDEX)
+    gen = Q._grid_call(n, n + 1, nz, 0., (n - 1) * d, 0., n * d, 0., (nz - 1) * d, lam, alphas,
+                       64, _lib.DW_INDEX)
+    for a, b in zip(sym, gen):
+        a, b = _lib.to_numpy(a), _lib.to_numpy(b)
+        assert np.array_equal(a, b[:, :n, :])
+    ex_ref = P.debye_wolf_grid(n, n, nz, 0., (n - 1) * d, 0., (n - 1) * d, 0., (nz - 1) * d,
+                               lam, P._alphas(1.1, 1.33), 64)[0]
+    assert O.rel_l2(_lib.to_numpy(sym[1]), ex_ref) < TOL
