@@ -170,46 +170,43 @@ __global__ void __launch_bounds__(256) k_dw_samples(DwArgs a, int total, int nco
   if (col == 0) co[s] = sm.co;
 }
 
-// The same table when the x and y axes are identical (nx == ny, x1 == y1,
-// x2 == y2, as for the octant of a centred grid): kr(i, j) and kr(j, i) are
-// then bit-identical (the same coordinates, and x^2 + y^2 rounds the same in
-// either order), so one thread evaluates the Bessel terms for the pair i <= j
-// and writes both columns.  Threads past the npairs pairs zero the padding
-// columns [ncols, cp); rows s >= total are zero.
-__global__ void __launch_bounds__(256) k_dw_samples_sym(DwArgs a, int total, int ncols, int tp, int cp,
-                                                        int npairs, const float* __restrict__ par,
-                                                        float* co, float* A) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+// Column pair p -> (i, j), i <= j, in the order (0,0), (0,1), (1,1), (0,2), ...
+BPMH_DEV void pair_ij(int p, int& i, int& j) {
+  j = (int)((sqrtf(8.f * (float)p + 1.f) - 1.f) * 0.5f);
+  while (j * (j + 1) / 2 > p) --j;
+  while ((j + 1) * (j + 2) / 2 <= p) ++j;
+  i = p - j * (j + 1) / 2;
+}
+
+// The table indexed by column pairs, for identical x and y axes (nx == ny,
+// x1 == y1, x2 == y2, as for the octant of a centred grid): kr(i, j) and
+// kr(j, i) are then bit-identical (the same coordinates, and x^2 + y^2
+// rounds the same in either order), so column p holds the amplitudes of
+// both lateral points (i, j) and (j, i) of pair p; npairs = nx (nx + 1) / 2
+// columns, zero-padded to cp.
+__global__ void __launch_bounds__(256) k_dw_samples_pairs(DwArgs a, int total, int npairs, int tp, int cp,
+                                                          const float* __restrict__ par, float* co,
+                                                          float* A) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
   const int s = blockIdx.y;
-  const int npad = cp - ncols;
-  if (idx >= npairs + npad) return;
+  if (col >= cp) return;
   const size_t plane = (size_t)tp * cp;
-  float* row = A + (size_t)s * cp;
-  if (idx >= npairs) {  // padding column
-    const int col = ncols + (idx - npairs);
-    row[col] = row[plane + col] = row[2 * plane + col] = 0.f;
+  const size_t o = (size_t)s * cp + col;
+  if (s >= total || col >= npairs) {
+    A[o] = A[plane + o] = A[2 * plane + o] = 0.f;
+    if (col == 0) co[s] = 0.f;
     return;
   }
-  // pair idx -> (i, j), i <= j, in the order (0,0), (0,1), (1,1), (0,2), ...
-  int j = (int)((sqrtf(8.f * (float)idx + 1.f) - 1.f) * 0.5f);
-  while (j * (j + 1) / 2 > idx) --j;
-  while ((j + 1) * (j + 2) / 2 <= idx) ++j;
-  const int i = idx - j * (j + 1) / 2;
-  const int c1 = j * a.nx + i, c2 = i * a.nx + j;
-  if (s >= total) {
-    row[c1] = row[plane + c1] = row[2 * plane + c1] = 0.f;
-    row[c2] = row[plane + c2] = row[2 * plane + c2] = 0.f;
-    if (idx == 0) co[s] = 0.f;
-    return;
-  }
+  int i, j;
+  pair_ij(col, i, j);
   const float x = axis_coord(i, a.nx, a.x1, a.x2);
   const float y = axis_coord(j, a.ny, a.y1, a.y2);
   const float kr = k_times(a.lam, __fsqrt_rn(f_add(f_mul(x, x), f_mul(y, y))));
   const Sample sm = sample_amps(SampleParams{par[3 * s], par[3 * s + 1], par[3 * s + 2]}, kr);
-  row[c1] = row[c2] = sm.a0;
-  row[plane + c1] = row[plane + c2] = sm.a1;
-  row[2 * plane + c1] = row[2 * plane + c2] = sm.a2;
-  if (idx == 0) co[s] = sm.co;
+  A[o] = sm.a0;
+  A[plane + o] = sm.a1;
+  A[2 * plane + o] = sm.a2;
+  if (col == 0) co[s] = sm.co;
 }
 
 // P[s][z] = e^{i kz cos t_s}, with the float32 argument kz*cos t; zero-padded
@@ -245,7 +242,7 @@ typedef float dm_v4f __attribute__((ext_vector_type(4)));
 
 // ZT 16-plane tiles (consecutive, from zt0) x DM_CT 16-column tiles per
 // wavefront; each loaded operand feeds ZT (left) or DM_CT (right) MFMAs.
-template <int ZT>
+template <int ZT, bool PAIRS>
 BPMH_DEV void dw_mfma_tiles(const DwArgs& a, int ncols, int zp, int tp, int cp, int zt0, int c0, int l,
                             const float2* __restrict__ P, const float* __restrict__ A) {
   const int lr = l & 15, lk = l >> 4;
@@ -305,24 +302,34 @@ BPMH_DEV void dw_mfma_tiles(const DwArgs& a, int ncols, int zp, int tp, int cp, 
           }
   }
 
+  // table column c is lateral point c, or (PAIRS) the points (i, j) and
+  // (j, i) of pair c, each combined with its own azimuth
   const size_t out_plane = (size_t)a.nx * a.ny;
 #pragma unroll
   for (int t = 0; t < DM_CT; ++t) {
     const int c = c0 + 16 * t + lr;
     if (c >= ncols) continue;
-    const int i = c % a.nx, j = c / a.nx;
-    const float x = axis_coord(i, a.nx, a.x1, a.x2);
-    const float y = axis_coord(j, a.ny, a.y1, a.y2);
-    const float phi = (float)atan2((double)y, (double)x);
+    int i, j;
+    if (PAIRS) pair_ij(c, i, j);
+    else i = c % a.nx, j = c / a.nx;
+    const bool second = PAIRS && i != j;
+    const float phi1 = (float)atan2((double)axis_coord(j, a.ny, a.y1, a.y2),
+                                    (double)axis_coord(i, a.nx, a.x1, a.x2));
+    const float phi2 = second ? (float)atan2((double)axis_coord(i, a.ny, a.y1, a.y2),
+                                             (double)axis_coord(j, a.nx, a.x1, a.x2))
+                              : 0.f;
+    const size_t o1 = (size_t)j * a.nx + i, o2 = (size_t)i * a.nx + j;
 #pragma unroll
     for (int r = 0; r < ZT; ++r)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int z = (zt0 + r) * 16 + 4 * lk + q;
-        if (z < a.nz)
-          combine(a, phi, make_float2(acc[r][t][0][q], acc[r][t][1][q]),
-                  make_float2(acc[r][t][2][q], acc[r][t][3][q]),
-                  make_float2(acc[r][t][4][q], acc[r][t][5][q]), (size_t)z * out_plane + c);
+        if (z >= a.nz) continue;
+        const float2 I0 = make_float2(acc[r][t][0][q], acc[r][t][1][q]);
+        const float2 I1 = make_float2(acc[r][t][2][q], acc[r][t][3][q]);
+        const float2 I2 = make_float2(acc[r][t][4][q], acc[r][t][5][q]);
+        combine(a, phi1, I0, I1, I2, (size_t)z * out_plane + o1);
+        if (second) combine(a, phi2, I0, I1, I2, (size_t)z * out_plane + o2);
       }
   }
 }
@@ -330,15 +337,17 @@ BPMH_DEV void dw_mfma_tiles(const DwArgs& a, int ncols, int zp, int tp, int cp, 
 // Wavefront w of work-group (x, y) owns the plane-tile pair 4y + w (16-plane
 // tiles 2(4y+w) and 2(4y+w)+1, the second only if it exists) and the
 // lateral points 32x .. 32x+31.
-__global__ void __launch_bounds__(256) k_dw_mfma(DwArgs a, int ncols, int nzt, int tp, int cp,
+// At most 256 registers per lane (two wavefronts per SIMD).
+template <bool PAIRS>
+__global__ void __launch_bounds__(256, 2) k_dw_mfma(DwArgs a, int ncols, int nzt, int tp, int cp,
                                                 const float2* __restrict__ P,
                                                 const float* __restrict__ A) {
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int zt0 = 2 * (blockIdx.y * 4 + w);
   if (zt0 >= nzt) return;  // whole wavefront; the kernel has no barriers
   const int c0 = blockIdx.x * (16 * DM_CT);
-  if (zt0 + 1 < nzt) dw_mfma_tiles<2>(a, ncols, nzt * 16, tp, cp, zt0, c0, l, P, A);
-  else dw_mfma_tiles<1>(a, ncols, nzt * 16, tp, cp, zt0, c0, l, P, A);
+  if (zt0 + 1 < nzt) dw_mfma_tiles<2, PAIRS>(a, ncols, nzt * 16, tp, cp, zt0, c0, l, P, A);
+  else dw_mfma_tiles<1, PAIRS>(a, ncols, nzt * 16, tp, cp, zt0, c0, l, P, A);
 }
 
 __global__ void __launch_bounds__(64) k_debye_points(DwArgs a, int n) {
@@ -508,9 +517,12 @@ extern "C" int bpmh_debye_wolf(int nx, int ny, int nz, float x1, float x2, float
   const int total = (int)total_ll;
   // stream-ordered scratch, zero-padded to whole product tiles: A (3 x tp x
   // cp), co (tp), P (tp x zp), per-sample parameters (3 x total)
+  // identical x and y axes: the table and the product run over column pairs
+  const bool pairs = nx == ny && x1 == y1 && x2 == y2;
+  const int ntab = pairs ? nx * (nx + 1) / 2 : ncols;  // table columns
   const int tp = (total + 15) / 16 * 16;
   const int nzt = (nz + 15) / 16, zp = nzt * 16;
-  const int cp = (ncols + 16 * DM_CT - 1) / (16 * DM_CT) * (16 * DM_CT);
+  const int cp = (ntab + 16 * DM_CT - 1) / (16 * DM_CT) * (16 * DM_CT);
   const size_t nA = (size_t)3 * tp * cp, nP = (size_t)tp * zp;
   void* scr = nullptr;
   hipMemPool_t pool = nullptr;
@@ -525,18 +537,20 @@ extern "C" int bpmh_debye_wolf(int nx, int ny, int nz, float x1, float x2, float
   float* co = (float*)(P + nP);
   float* par = co + tp;
   hipLaunchKernelGGL(k_dw_params, dim3((total + 63) / 64), dim3(64), 0, st, a, total, par);
-  const bool sym = nx == ny && x1 == y1 && x2 == y2;
-  if (sym) {
-    const int npairs = nx * (nx + 1) / 2;
-    hipLaunchKernelGGL(k_dw_samples_sym, dim3((npairs + (cp - ncols) + 255) / 256, tp), dim3(256), 0, st, a,
-                       total, ncols, tp, cp, npairs, (const float*)par, co, A);
-  } else {
+  if (pairs)
+    hipLaunchKernelGGL(k_dw_samples_pairs, dim3((cp + 255) / 256, tp), dim3(256), 0, st, a, total, ntab, tp,
+                       cp, (const float*)par, co, A);
+  else
     hipLaunchKernelGGL(k_dw_samples, dim3((cp + 255) / 256, tp), dim3(256), 0, st, a, total, ncols, tp, cp,
                        (const float*)par, co, A);
-  }
   hipLaunchKernelGGL(k_dw_phases, dim3((zp + 63) / 64, tp), dim3(64), 0, st, a, total, tp, zp, (const float*)co, P);
-  hipLaunchKernelGGL(k_dw_mfma, dim3(cp / (16 * DM_CT), ((nzt + 1) / 2 + 3) / 4), dim3(256), 0, st, a, ncols, nzt, tp, cp,
-                     (const float2*)P, (const float*)A);
+  const dim3 mgrid(cp / (16 * DM_CT), ((nzt + 1) / 2 + 3) / 4);
+  if (pairs)
+    hipLaunchKernelGGL(k_dw_mfma<true>, mgrid, dim3(256), 0, st, a, ntab, nzt, tp, cp, (const float2*)P,
+                       (const float*)A);
+  else
+    hipLaunchKernelGGL(k_dw_mfma<false>, mgrid, dim3(256), 0, st, a, ncols, nzt, tp, cp, (const float2*)P,
+                       (const float*)A);
   e = hipGetLastError();
   (void)hipFreeAsync(scr, st);
   if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(e));
