@@ -228,15 +228,17 @@ __global__ void __launch_bounds__(64) k_dw_phases(DwArgs a, int total, int tp, i
 // lane l holds A[l%16][l/16] of the 16x4 left operand, B[l/16][l%16] of the
 // 4x16 right operand, and D[4(l/16)+j][l%16], j < 4, of the 16x16 result).
 // Left operand: Re or Im of P for 16 planes x 4 samples; right operand: A_n
-// for 4 samples x 16 lateral points.  Each wavefront owns up to 2 tiles of
-// 16 planes x DM_CT tiles of 16 lateral points (6 accumulators per tile
-// pair: Re, Im of I0, I1, I2) and reads its operands straight from the
-// tables into registers, the next 16 samples in flight while the current
-// ones are multiplied; no LDS and no barriers, so wavefronts run
-// independently.  The four wavefronts of a work-group share their lateral
-// points (their right operands hit in L1).  (Measured alternatives: one
-// plane tile per wavefront 3 % slower; sample-contiguous tables with one
-// 16-byte operand load per row and batch 6 % slower.)
+// for 4 samples x 16 lateral points.  Each wavefront owns ZT tiles of 16
+// planes (the kernel launches ZT = 1) x DM_CT tiles of 16 table columns (6
+// accumulators per tile pair: Re, Im of I0, I1, I2) and reads its operands
+// straight from the tables into registers, the next 16 samples in flight
+// while the current ones are multiplied; no LDS and no barriers, so
+// wavefronts run independently.  The four wavefronts of a work-group share
+// their table columns (their right operands hit in L1).  (Measured
+// alternatives at 256^3 with the pair table: two plane tiles per wavefront
+// 6 % slower, one column tile per wavefront 5 % slower; earlier,
+// sample-contiguous tables with one 16-byte operand load per row and batch
+// were 6 % slower.)
 constexpr int DM_CT = 2;  // 16-column tiles per wavefront
 typedef float dm_v4f __attribute__((ext_vector_type(4)));
 
@@ -334,20 +336,20 @@ BPMH_DEV void dw_mfma_tiles(const DwArgs& a, int ncols, int zp, int tp, int cp, 
   }
 }
 
-// Wavefront w of work-group (x, y) owns the plane-tile pair 4y + w (16-plane
-// tiles 2(4y+w) and 2(4y+w)+1, the second only if it exists) and the
-// lateral points 32x .. 32x+31.
-// At most 256 registers per lane (two wavefronts per SIMD).
+// Wavefront w of work-group (x, y) owns the 16-plane tile 4y + w and the
+// table columns 32x .. 32x+31.  (Two plane tiles per wavefront reuse each
+// loaded column operand twice but need 220 registers, two wavefronts per
+// SIMD, and leave a tail of unequal wavefronts: 0.364 vs 0.341 ms per 256^3
+// call.)
 template <bool PAIRS>
 __global__ void __launch_bounds__(256, 2) k_dw_mfma(DwArgs a, int ncols, int nzt, int tp, int cp,
                                                 const float2* __restrict__ P,
                                                 const float* __restrict__ A) {
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int zt0 = 2 * (blockIdx.y * 4 + w);
+  const int zt0 = blockIdx.y * 4 + w;
   if (zt0 >= nzt) return;  // whole wavefront; the kernel has no barriers
   const int c0 = blockIdx.x * (16 * DM_CT);
-  if (zt0 + 1 < nzt) dw_mfma_tiles<2, PAIRS>(a, ncols, nzt * 16, tp, cp, zt0, c0, l, P, A);
-  else dw_mfma_tiles<1, PAIRS>(a, ncols, nzt * 16, tp, cp, zt0, c0, l, P, A);
+  dw_mfma_tiles<1, PAIRS>(a, ncols, nzt * 16, tp, cp, zt0, c0, l, P, A);
 }
 
 __global__ void __launch_bounds__(64) k_debye_points(DwArgs a, int n) {
@@ -544,7 +546,7 @@ extern "C" int bpmh_debye_wolf(int nx, int ny, int nz, float x1, float x2, float
     hipLaunchKernelGGL(k_dw_samples, dim3((cp + 255) / 256, tp), dim3(256), 0, st, a, total, ncols, tp, cp,
                        (const float*)par, co, A);
   hipLaunchKernelGGL(k_dw_phases, dim3((zp + 63) / 64, tp), dim3(64), 0, st, a, total, tp, zp, (const float*)co, P);
-  const dim3 mgrid(cp / (16 * DM_CT), ((nzt + 1) / 2 + 3) / 4);
+  const dim3 mgrid(cp / (16 * DM_CT), (nzt + 3) / 4);
   if (pairs)
     hipLaunchKernelGGL(k_dw_mfma<true>, mgrid, dim3(256), 0, st, a, ntab, nzt, tp, cp, (const float2*)P,
                        (const float*)A);
