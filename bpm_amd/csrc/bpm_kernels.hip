@@ -291,11 +291,13 @@ k_row(RowArgs a) {
         } else {
           r = c_mul(v[m], make_float2(c, s));
         }
-        const int dist = a.absorb > 0 ? min(min(NX - x - 1, x), disty) : 0;
-        if (dist < a.absorb) {
-          const float arg = f_div(f_mul(BPMH_PI_F, (float)dist), (float)a.absorb);
-          const float tp = f_mul(0.5f, f_sub(1.f, (float)cos((double)arg)));
-          r = make_float2(r.x * tp, r.y * tp);
+        if (a.absorb > 0) {  // uniform: no taper work at all without one
+          const int dist = min(min(NX - x - 1, x), disty);
+          if (dist < a.absorb) {
+            const float arg = f_div(f_mul(BPMH_PI_F, (float)dist), (float)a.absorb);
+            const float tp = f_mul(0.5f, f_sub(1.f, (float)cos((double)arg)));
+            r = make_float2(r.x * tp, r.y * tp);
+          }
         }
         v[m] = r;
       }
