@@ -1,4 +1,5 @@
-// Bessel functions of the first kind J_0, J_1, J_2 in float64, host and device.
+// Bessel functions of the first kind J_0, J_1, J_2 in float64 (j012, jn) and
+// for float32 arguments and results (j012_f32), host and device.
 //
 // Methods (Abramowitz & Stegun, Handbook of Mathematical Functions, 1964):
 //   |x| <  14: ascending power series, A&S 9.1.10
@@ -14,11 +15,46 @@
 // The coefficients (exact rationals rounded once to float64) are printed by
 // gen_bessel_coefs.py.  Polynomials have fixed degree, so there are no
 // data-dependent loops or divisions.  J_n(-x) = (-1)^n J_n(x).
+// j012_f32 keeps the float64 series below 14 and evaluates the expansion in
+// float32 above it, with 6 terms of P and Q.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
 
 namespace bpmbessel {
+
+// Hankel asymptotic expansion coefficients of P_n and Q_n in 1/x^2 (A&S
+// 9.2.9/9.2.10 for n = 0, 1, 2), exact rationals rounded once to float64.
+constexpr double P0[12] = {
+    1.0, -0.0703125, 0.112152099609375,
+    -0.5725014209747314, 6.074042001273483, -110.01714026924674,
+    3038.090510922384, -118838.42625678325, 6252951.493434797,
+    -425939216.5047669, 36468400807.06556, -3833534661393.9443};
+constexpr double Q0[12] = {
+    -0.125, 0.0732421875, -0.22710800170898438,
+    1.7277275025844574, -24.380529699556064, 551.3358961220206,
+    -18257.755474293175, 832859.3040162893, -50069589.531988926,
+    3836255180.2304335, -364901081884.98334, 42189715702840.97};
+constexpr double P1[12] = {
+    1.0, 0.1171875, -0.144195556640625,
+    0.6765925884246826, -6.883914268109947, 121.59789187653587,
+    -3302.2722944808525, 127641.2726461746, -6656367.718817688,
+    450278600.3050393, -38338575207.427895, 4011838599133.1978};
+constexpr double Q1[12] = {
+    0.375, -0.1025390625, 0.2775764465332031,
+    -1.993531733751297, 27.248827311268542, -603.8440767050702,
+    19718.37591223663, -890297.8767070678, 53104110.10968523,
+    -4043620325.107754, 382701134659.8606, -44064814178522.79};
+constexpr double P2[12] = {
+    1.0, -0.8203125, 0.317230224609375,
+    -1.1276543140411377, 10.061105468776077, -164.51479489178382,
+    4245.778664332524, -158275.1780812565, 8033547.2468489325,
+    -532147436.72413737, 44555641457.28107, -4598936930713.666};
+constexpr double Q2[12] = {
+    1.875, 0.3076171875, -0.5154991149902344,
+    3.0809126794338226, -38.14835823577596, 794.5316798750923,
+    -24862.30006325488, 1088141.8493086386, -63382324.9696243,
+    4736812380.840512, -441578232299.8392, 50213392901107.36};
 
 template <int K>
 __host__ __device__ inline double horner(const double (&c)[K], double q) {
@@ -72,36 +108,6 @@ __host__ __device__ inline void j012_abs(double ax, double& j0, double& j1, doub
     j1 = h * horner(S1, q);
     j2 = h * h * horner(S2, q);
   } else {
-    constexpr double P0[12] = {
-        1.0, -0.0703125, 0.112152099609375,
-        -0.5725014209747314, 6.074042001273483, -110.01714026924674,
-        3038.090510922384, -118838.42625678325, 6252951.493434797,
-        -425939216.5047669, 36468400807.06556, -3833534661393.9443};
-    constexpr double Q0[12] = {
-        -0.125, 0.0732421875, -0.22710800170898438,
-        1.7277275025844574, -24.380529699556064, 551.3358961220206,
-        -18257.755474293175, 832859.3040162893, -50069589.531988926,
-        3836255180.2304335, -364901081884.98334, 42189715702840.97};
-    constexpr double P1[12] = {
-        1.0, 0.1171875, -0.144195556640625,
-        0.6765925884246826, -6.883914268109947, 121.59789187653587,
-        -3302.2722944808525, 127641.2726461746, -6656367.718817688,
-        450278600.3050393, -38338575207.427895, 4011838599133.1978};
-    constexpr double Q1[12] = {
-        0.375, -0.1025390625, 0.2775764465332031,
-        -1.993531733751297, 27.248827311268542, -603.8440767050702,
-        19718.37591223663, -890297.8767070678, 53104110.10968523,
-        -4043620325.107754, 382701134659.8606, -44064814178522.79};
-    constexpr double P2[12] = {
-        1.0, -0.8203125, 0.317230224609375,
-        -1.1276543140411377, 10.061105468776077, -164.51479489178382,
-        4245.778664332524, -158275.1780812565, 8033547.2468489325,
-        -532147436.72413737, 44555641457.28107, -4598936930713.666};
-    constexpr double Q2[12] = {
-        1.875, 0.3076171875, -0.5154991149902344,
-        3.0809126794338226, -38.14835823577596, 794.5316798750923,
-        -24862.30006325488, 1088141.8493086386, -63382324.9696243,
-        4736812380.840512, -441578232299.8392, 50213392901107.36};
     const double y = 1.0 / ax, y2 = y * y;
     const double amp = sqrt(2.0 / (M_PI * ax));
     double s, c;
@@ -113,6 +119,54 @@ __host__ __device__ inline void j012_abs(double ax, double& j0, double& j1, doub
     j1 = amp * (p1 * s + q1 * c);     // cos chi_1 = sin chi_0, sin chi_1 = -cos chi_0
     j2 = -amp * (p2 * c - q2 * s);    // cos chi_2 = -cos chi_0, sin chi_2 = -sin chi_0
   }
+}
+
+// The first K coefficients of c in float32 Horner form.
+template <int K, int N>
+__host__ __device__ inline float horner_f32(const double (&c)[N], float q) {
+  static_assert(K <= N, "not enough coefficients");
+  float r = (float)c[K - 1];
+#pragma unroll
+  for (int k = K - 2; k >= 0; --k) r = fmaf(r, q, (float)c[k]);
+  return r;
+}
+
+// J_0, J_1, J_2 at ax >= 14 in float32: the same Hankel expansion with the
+// first 6 terms of P and Q (the omitted ones are below 2e-9 of the leading
+// term at x = 14), and the phase chi_0 = x - pi/4 taken from sincosf of the
+// exact argument rotated by pi/4 (sin chi_0 = (sin x - cos x)/sqrt 2,
+// cos chi_0 = (cos x + sin x)/sqrt 2), so no rounding of x - pi/4 enters.
+// Within a few float32 ulp of the float64 evaluation.
+__host__ __device__ inline void j012_asym_f32(float ax, float& j0, float& j1, float& j2) {
+  const float y = 1.f / ax, y2 = y * y;
+  const float amp = sqrtf(0.636619772f * y);  // sqrt(2 / (pi x))
+  float sx, cx;
+  sincosf(ax, &sx, &cx);
+  const float r2 = 0.707106781f;
+  const float s = (sx - cx) * r2, c = (cx + sx) * r2;
+  const float p0 = horner_f32<6>(P0, y2), q0 = y * horner_f32<6>(Q0, y2);
+  const float p1 = horner_f32<6>(P1, y2), q1 = y * horner_f32<6>(Q1, y2);
+  const float p2 = horner_f32<6>(P2, y2), q2 = y * horner_f32<6>(Q2, y2);
+  j0 = amp * (p0 * c - q0 * s);
+  j1 = amp * (p1 * s + q1 * c);
+  j2 = -amp * (p2 * c - q2 * s);
+}
+
+// J_0, J_1, J_2 of a float32 argument as float32: the float64 power series
+// below 14 (where float32 would lose digits to cancellation), the float32
+// expansion above.
+__host__ __device__ inline void j012_f32(float x, float& j0, float& j1, float& j2) {
+  const float ax = fabsf(x);
+  if (ax < 14.f) {
+    double d0, d1, d2;
+    j012_abs((double)ax, d0, d1, d2);
+    j0 = (float)d0;
+    j1 = (float)d1;
+    j2 = (float)d2;
+  } else {
+    j012_asym_f32(ax, j0, j1, j2);
+  }
+  if (x < 0.f) j1 = -j1;
 }
 
 // J_0(x), J_1(x), J_2(x)

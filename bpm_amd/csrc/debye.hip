@@ -91,14 +91,13 @@ BPMH_DEV SampleParams sample_params(const DwArgs& a, int i_n, int i_t) {
 
 // amplitudes w (cos t + 1) J0, w sin t J1, w (cos t - 1) J2 at kr sin t
 BPMH_DEV Sample sample_amps(SampleParams p, float kr) {
-  const double ks = (double)f_mul(kr, p.si);
-  double j0, j1, j2;
-  bpmbessel::j012(ks, j0, j1, j2);
+  float j0, j1, j2;
+  bpmbessel::j012_f32(f_mul(kr, p.si), j0, j1, j2);
   Sample s;
   s.co = p.co;
-  s.a0 = f_mul(f_mul(p.w, f_add(p.co, 1.f)), (float)j0);
-  s.a1 = f_mul(f_mul(p.w, p.si), (float)j1);
-  s.a2 = f_mul(f_mul(p.w, f_sub(p.co, 1.f)), (float)j2);
+  s.a0 = f_mul(f_mul(p.w, f_add(p.co, 1.f)), j0);
+  s.a1 = f_mul(f_mul(p.w, p.si), j1);
+  s.a2 = f_mul(f_mul(p.w, f_sub(p.co, 1.f)), j2);
   return s;
 }
 
@@ -424,9 +423,13 @@ __global__ void __launch_bounds__(128) k_debye_mirror2(int nx0, int ny0, int nz0
   }
 }
 
+// the float32 evaluation that the Debye sample tables use
 __global__ void k_bessel_fill(int n, const float* x, float* out, int count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) out[i] = (float)bpmbessel::jn(n, (double)x[i]);
+  if (i >= count) return;
+  float j0, j1, j2;
+  bpmbessel::j012_f32(x[i], j0, j1, j2);
+  out[i] = n == 0 ? j0 : (n == 1 ? j1 : j2);
 }
 
 static int fill_args(DwArgs& a, float lam, const float* alphas, int n_alphas, int int_steps,

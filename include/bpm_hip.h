@@ -196,8 +196,8 @@ int bpmh_cylindrical(int ny, int nz, float y1, float y2, float z1, float z2, flo
                      float alpha, int int_steps, int plane_variant, float* ex, float* ey,
                      float* ez, float* I, void* stream);
 
-/* out[i] = J_n(x[i]) for n in {0,1,2} with this library's Bessel functions
- * (device arrays).  Replaces the bessel_fill self-test kernel,
+/* out[i] = J_n(x[i]) for n in {0,1,2} with this library's float32 Bessel
+ * evaluation, the one its Debye tables use (device arrays).  Replaces the bessel_fill self-test kernel,
  * bpm/psf/kernels/bessel.cl:120-129 (bpm/psf/_focus_fields_debye.py:34-42). */
 int bpmh_bessel_fill(int n, const float* x, float* out, int count, void* stream);
 
