@@ -1,7 +1,7 @@
 """GPU parity of the HIP split-step BPM (through the C ABI) against the oracle.
 
-Bar: complex64 fields and intensities within relative
-L2 <= 1e-4 of the complex64 restatement of the reference float32 path; the
+Bar: complex64 fields and intensities within relative L2 <= 1e-4 of the
+complex64 restatement of the reference float32 path; the
 error against the float64 oracle is printed alongside.  Full-size cases are
 checked through size-independent properties (closed form, Parseval,
 batch == independent runs).
@@ -328,7 +328,7 @@ def test_launch_variants_agree(bpm, nx, ny):
 
 
 @pytest.mark.parametrize("config", ["256x256x256", "1024x1024x1024"])
-def test_baseline_configs_vs_c64_and_f64_oracles(bpm, config):
+def test_plain_workloads_vs_c64_and_f64_oracles(bpm, config):
     """The plain workloads of the benchmark (Bpm3d path: float64 propagator
     rounded to complex64, real dn, exit field + sum_z |u_z|^2) in full:
     256^2 x 256 (21 spheres, plane wave) and 1024^2 x 1024 (400 spheres,
