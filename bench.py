@@ -257,11 +257,11 @@ def small_extra(torch, seed):
 def batched_extra(torch, seed, dn_g, units, lam, n0):
     """Batched form at 1024^2: B fields (focused Gaussian beams tilted by
     seeded plane-wave phases) through the first 129 planes of the benchmark
-    dn in one launch sequence per pass, graph-replayed; per-step time is
-    independent of depth."""
+    dn (all of them if it has fewer) in one launch sequence per pass,
+    graph-replayed; per-step time is independent of depth."""
     from bpm_amd import _lib
     from bpm_amd import bpm3d as B
-    nz = 129
+    nz = min(129, dn_g.shape[0])
     ny, nx = dn_g.shape[1], dn_g.shape[2]
     H = B.propagator_table(nx, ny, units, lam, n0, _lib.H_F64)
     rng = np.random.default_rng(seed)

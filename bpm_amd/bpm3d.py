@@ -77,6 +77,56 @@ def propagator_table(Nx, Ny, units, lam, n0, kind=_lib.H_F32):
     return H
 
 
+def _check_operands(plane, nz, dn, dn_kind, dn_batch_stride, mm, H, n_eff, dn0, out_field,
+                    out_intens, out_sum, sw, gw, plain, sout):
+    """Every tensor bpmh_propagate will touch, against what its kernels read
+    and write for this batch, depth and grid (the C ABI sees bare pointers):
+    raises ValueError before anything is launched."""
+    import torch as t  # dtypes only; the device is checked last
+    if plane.dim() != 3 or plane.dtype != t.complex64 or not plane.is_contiguous():
+        raise ValueError("plane must be a contiguous (B, Ny, Nx) complex64 tensor")
+    B, Ny, Nx = plane.shape
+    n = Ny * Nx
+    if nz < 1:
+        raise ValueError("nz must be >= 1")
+    if dn_batch_stride < 0:
+        raise ValueError("dn_batch_stride must be >= 0")
+
+    def need(x, name, dtype, numel):
+        if x is None:
+            return
+        if x.dtype != dtype or not x.is_contiguous():
+            raise ValueError("%s must be a contiguous %s tensor" % (name, dtype))
+        if x.numel() < numel:
+            raise ValueError("%s holds %d elements; this propagation reads or writes %d"
+                             % (name, x.numel(), numel))
+        if x.device != plane.device:
+            raise ValueError("%s must be on %s" % (name, plane.device))
+
+    if dn_kind != _lib.DN_NONE:
+        if dn is None:
+            raise ValueError("dn_kind needs a dn tensor")
+        kind_dtype = {_lib.DN_F32: t.float32, _lib.DN_F16: t.float16, _lib.DN_C64: t.complex64}
+        if dn_kind not in kind_dtype:
+            raise ValueError("unknown dn kind")
+        # planes 0..nz-1 of every item (slice 0 feeds the weighted mean)
+        need(dn, "dn", kind_dtype[dn_kind], (B - 1) * dn_batch_stride + nz * n)
+    if mm == _lib.MEAN_IGNORE:
+        need(H, "H", t.complex64, n)
+    if mm == _lib.MEAN_SIMPLE:
+        need(n_eff, "n_eff", t.float32, nz)
+        need(dn0, "dn0", t.float32, nz)
+    need(out_field, "out_field", t.complex64, B * nz * n)
+    need(out_intens, "out_intens", t.float32, B * nz * n)
+    need(out_sum, "out_sum", t.float32, B * n)
+    need(sw, "scattering weights", t.float32, n)
+    need(gw, "g weights", t.float32, n)
+    need(plain, "plane-wave array", t.complex64, nz)
+    need(sout, "scattering output", t.float32, B * nz * 2)
+    if not plane.is_cuda:
+        raise ValueError("plane must be a GPU tensor")
+
+
 def propagate_fields(plane, nz, units, lam, n0=1., dn=None, dn_kind=None, dn_batch_stride=0,
                      mean_mode="ignore", H=None, n_eff=None, dn0=None, absorbing_width=0,
                      out_field=None, out_intens=None, out_sum=None, scat=None, tuned=True):
@@ -102,6 +152,8 @@ def propagate_fields(plane, nz, units, lam, n0=1., dn=None, dn_kind=None, dn_bat
     sw = gw = plain = sout = None
     if scat is not None:
         sw, gw, plain, sout = scat
+    _check_operands(plane, nz, dn, dn_kind, dn_batch_stride, mm, H, n_eff, dn0,
+                    out_field, out_intens, out_sum, sw, gw, plain, sout)
     _lib.call("bpmh_propagate", get_plan(Nx, Ny, B if tuned else None), int(B), int(nz), _lib.ptr(plane), _lib.ptr(dn),
               int(dn_kind), int(dn_batch_stride), int(mm), _lib.ptr(H), _lib.ptr(n_eff),
               _lib.ptr(dn0), float(np.float32(n0)), float(np.float32(k0)), float(np.float32(dx)),

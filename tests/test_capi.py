@@ -83,3 +83,30 @@ def test_bessel_code_matches_scipy_on_host(lib):
         assert np.abs(out - ref).max() < 1e-10
         small = (np.abs(x) < 1) & (x != 0)
         assert np.allclose(out[small], ref[small], rtol=1e-13, atol=0)
+
+
+def test_propagate_fields_checks_operand_extents_before_launch():
+    """propagate_fields hands bare pointers to the library, so it checks every
+    tensor against what the passes read and write for this batch, depth and
+    grid first: a dn with fewer planes than nz, a short output volume or a
+    propagator of the wrong type raise ValueError before any device work
+    (CPU tensors here; the GPU check comes last)."""
+    import torch
+    from bpm_amd import bpm3d as B
+    plane = torch.zeros((2, 16, 16), dtype=torch.complex64)
+    H = torch.zeros((16, 16), dtype=torch.complex64)
+    dn = torch.zeros((3, 16, 16), dtype=torch.float32)
+    cases = [
+        (dict(dn=dn, dn_kind=_lib.DN_F32, H=H), 129, "dn holds"),  # 3 planes, 129 steps
+        (dict(dn=torch.zeros((2, 5, 16, 16)), dn_kind=_lib.DN_F32, dn_batch_stride=5 * 256, H=H), 6,
+         "dn holds"),
+        (dict(dn=dn.half(), dn_kind=_lib.DN_F32, H=H), 3, "dn must be"),
+        (dict(H=H.real.contiguous()), 3, "H must be"),
+        (dict(H=H, out_field=torch.zeros((2, 2, 16, 16), dtype=torch.complex64)), 3, "out_field holds"),
+        (dict(H=H, out_sum=torch.zeros((1, 16, 16))), 3, "out_sum holds"),
+        (dict(H=H, dn=dn, dn_kind=_lib.DN_F32, dn_batch_stride=-1), 3, "dn_batch_stride"),
+        (dict(H=H, dn=dn, dn_kind=_lib.DN_F32), 3, "GPU tensor"),  # all extents fine
+    ]
+    for kw, nz, msg in cases:
+        with pytest.raises(ValueError, match=msg):
+            B.propagate_fields(plane, nz, (.1, .1, .1), .5, 1.33, mean_mode="ignore", **kw)
