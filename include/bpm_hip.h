@@ -7,6 +7,10 @@
  * (re, im) pairs); work is enqueued on
  * the caller's hipStream_t (passed as void*, NULL = default stream) and is
  * asynchronous.  No entry point takes a host array except where stated.
+ * The library sees bare pointers and cannot check their extents: every array
+ * must hold what the declaration below says the call reads or writes for
+ * the given sizes (a short dn volume is read past its end, which faults the
+ * device).  bpm_amd.bpm3d.propagate_fields checks this for torch tensors.
  *
  * The reference binds its device work through gputools/pyopencl
  * (OCLProgram.run_kernel, fft, OCLReductionKernel); each declaration below
