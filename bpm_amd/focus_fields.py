@@ -84,8 +84,12 @@ def focus_field_debye_plane(shape=(128, 128), units=(.1, .1), z=0., lam=.5, NA=.
     dx, dy = units
     alphas = _alphas(NA, n0)
     t = _lib.torch()
-    if ex_g is not None and tuple(ex_g.shape) != (Ny, Nx):
-        raise AssertionError("ex_g must have shape (Ny,Nx)")
+    if ex_g is not None:
+        if tuple(ex_g.shape) != (Ny, Nx):
+            raise AssertionError("ex_g must have shape (Ny,Nx)")
+        # the kernel writes Ny*Nx complex64 values through the bare pointer
+        if ex_g.dtype != t.complex64 or not ex_g.is_contiguous() or not ex_g.is_cuda:
+            raise ValueError("ex_g must be a contiguous complex64 GPU tensor")
     out = ex_g if ex_g is not None else _lib.empty((Ny, Nx), np.complex64)
     a, ap = _alpha_arg(alphas)
     f = lambda v: float(np.float32(v))
@@ -203,8 +207,10 @@ def psf_u0(shape, units, zfoc=0, NA=.4, lam=.5, n0=1., n_integration_steps=200):
 
 
 def bessel_jn(n, x):
-    """J_n(x), n in {0,1,2}, evaluated on the GPU in float64 and returned as
-    float32 (the bessel_fill self-test of bpm/psf/kernels/bessel.cl:120-129)."""
+    """J_n(x), n in {0,1,2}, of float32 arguments on the GPU with the
+    evaluation the Debye tables use (float64 series below 14, float32
+    asymptotic expansion above), returned as float32 (the bessel_fill
+    self-test of bpm/psf/kernels/bessel.cl:120-129)."""
     xg = _lib.to_device(np.asarray(x, np.float32).ravel(), np.float32)
     out = _lib.empty(xg.shape, np.float32)
     _lib.call("bpmh_bessel_fill", int(n), _lib.ptr(xg), _lib.ptr(out), int(xg.numel()),
