@@ -187,3 +187,19 @@ def test_debye_symmetric_sample_table_is_bit_identical(bpm):
     ex_ref = P.debye_wolf_grid(n, n, nz, 0., (n - 1) * d, 0., (n - 1) * d, 0., (nz - 1) * d,
                                lam, P._alphas(1.1, 1.33), 64)[0]
     assert O.rel_l2(_lib.to_numpy(sym[1]), ex_ref) < TOL
+
+
+def test_plane_variant_fills_a_provided_tensor(bpm):
+    """focus_field_debye_plane(ex_g=...) writes the same bits into a given
+    (Ny, Nx) complex64 GPU tensor as it returns without one, and refuses a
+    tensor of another dtype before launching (the kernel writes complex64
+    through the bare pointer)."""
+    import torch
+    from bpm_amd import focus_fields as Q
+    say("case plane into ex_g")
+    ref = Q.focus_field_debye_plane((32, 24), (.1, .1), z=.2, NA=.6, n0=1.)
+    g = torch.zeros((24, 32), dtype=torch.complex64, device="cuda")
+    assert Q.focus_field_debye_plane((32, 24), (.1, .1), z=.2, NA=.6, n0=1., ex_g=g) is None
+    assert np.array_equal(g.cpu().numpy(), ref)
+    with pytest.raises(ValueError):
+        Q.focus_field_debye_plane((32, 24), (.1, .1), ex_g=torch.zeros((24, 32), device="cuda"))
