@@ -838,7 +838,6 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, 
   const col_fn col0 = pick_col<0>(ny, plan->col_v);
   const col_fn colA = pick_col<1>(ny, plan->col_v);
   const col_fn colB = pick_col<2>(ny, plan->col_v);
-  const int ntiles = col_tiles(nx, ny, plan->col_v);
 
   // scratch: weighted per-column partials (batch*nx) + scattering per-column
   // partials (batch*nz*nx)
