@@ -19,6 +19,7 @@
 #include "bpm_hip.h"
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -110,8 +111,13 @@ struct ColCfg : Base<N, col_e(N)> {
                                   ? 1 : (Base<N, col_e(N)>::E <= 4 ? 1024 : 512) / T;
   static constexpr int CMAX = pow2floor(clampi(CMAX0 < TMAX ? CMAX0 : TMAX, 1, 16));
   static constexpr int CMIN = 64 / T < 1 ? 1 : 64 / T;  // >= one full wavefront
-  static constexpr int COLS = clampi(scale_v(C0, V), CMIN, CMAX);
-  static constexpr int THREADS = COLS * T;
+  // V = 3: the tiles of V = 0 with P = 2 adjacent columns per thread, so a
+  // row of a column pair is one 16-byte load and store.  Used for columns
+  // of 256-1024 points (where it was measured); elsewhere it is V = 0.
+  static constexpr int P = (V == 3 && N >= 256 && N <= 1024) ? 2 : 1;
+  static constexpr int LC = clampi(scale_v(C0, V == 3 ? 0 : V), CMIN, CMAX);  // lane columns
+  static constexpr int COLS = LC * P;
+  static constexpr int THREADS = LC * T;
 };
 
 // ----------------------------------------------------------------------------
@@ -355,98 +361,132 @@ template <int NY, int V, int PHASE>
 __global__ void __launch_bounds__((ColCfg<NY, V>::THREADS))
 k_col(ColArgs a) {
   constexpr int E = ColCfg<NY, V>::E, T = ColCfg<NY, V>::T, COLS = ColCfg<NY, V>::COLS;
-  // swizzled lines 16/COLS float2 apart, or padded lines 9 float2 longer
-  // than padded: either way the COLS lines a 16-lane group touches together
+  // each thread holds P adjacent columns; the LC lane columns of a 16-lane
+  // group are the LDS lines it touches together
+  constexpr int P = ColCfg<NY, V>::P, LC = ColCfg<NY, V>::LC;
+  // swizzled lines 16/LC float2 apart, or padded lines 9 float2 longer
+  // than padded: either way the LC lines a 16-lane group touches together
   // start in different LDS banks
-  constexpr bool SWZ = COLS >= 4;
+  constexpr bool SWZ = LC >= 4;
   static_assert(COLS <= 16, "at most 16 columns per tile");
-  constexpr int LINE = SWZ ? NY + 16 / COLS : ColCfg<NY, V>::COL_LINE;
+  constexpr int LINE = SWZ ? NY + 16 / LC : ColCfg<NY, V>::COL_LINE;
   __shared__ float2 lds[COLS][LINE];
-  using CL = std::conditional_t<SWZ, SwzLayout<4 - ilog2c(COLS)>, PadLayout<ColCfg<NY, V>::COL_SH>>;
+  using CL = std::conditional_t<SWZ, SwzLayout<4 - ilog2c(LC)>, PadLayout<ColCfg<NY, V>::COL_SH>>;
   const int ntiles = a.nx / COLS;
   const int bx = blockIdx.x;
   // adjacent tiles share cache lines: keep them on one XCD (blocks are
   // dealt round-robin over the 8 XCDs)
   const int tile = (ntiles % 8 == 0) ? (bx & 7) * (ntiles >> 3) + (bx >> 3) : bx;
   const int b = blockIdx.y;
-  const int c = threadIdx.x % COLS;
-  const int t = threadIdx.x / COLS;
-  const int x = tile * COLS + c;
-  float2* p = a.plane + (size_t)b * NY * a.nx + x;
+  const int c = threadIdx.x % LC;
+  const int t = threadIdx.x / LC;
+  const int x0 = tile * COLS + c * P;  // the thread's columns x0 .. x0+P-1
+  float2* p = a.plane + (size_t)b * NY * a.nx + x0;
 
-  // all global loads first (plane, twiddles, the propagator quadrant)
-  float2 v[E];
+  // all global loads first (plane, twiddles, the propagator quadrant); with
+  // P = 2 the host guarantees a 16-byte aligned plane (x0 and nx are even)
+  float2 v[P][E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = p[(size_t)(t + m * T) * a.nx];
+  for (int m = 0; m < E; ++m) {
+    float2* r = p + (size_t)(t + m * T) * a.nx;
+    if constexpr (P == 2) {
+      const float4 w = *reinterpret_cast<const float4*>(r);
+      v[0][m] = make_float2(w.x, w.y);
+      v[1][m] = make_float2(w.z, w.w);
+    } else {
+      v[0][m] = *r;
+    }
+  }
   Twiddles<NY, E> tw;
   tw.load(t, a.tw);
   // H depends on kx^2 and ky^2 only, so H[j][i] = H[(ny-j)%ny][(nx-i)%nx]
   // holds bit for bit; reading the quadrant j<=ny/2, i<=nx/2 keeps the
   // table's footprint at a quarter (L2-resident per XCD strip).
-  const int qx = min(x, a.nx - x);
-  float2 hv[E];
+  float2 hv[P][E];
   if (PHASE != 1 && a.hmode == 0) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int ky = t + m * T;
-      hv[m] = a.H[(size_t)min(ky, NY - ky) * a.nx + qx];
-    }
-  }
-
-  if constexpr (PHASE != 2) line_fft<NY, E, false, CL>(v, lds[c], t, tw);
-
-  if constexpr (PHASE == 1) {
-    const size_t s0 = (size_t)b * a.dn_bstride;
-    float2 acc = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int ky = t + m * T;
-      const float au = hypotf(v[m].x, v[m].y);
-      const float d = a.dn ? load_dn(a.dn, a.dn_kind, s0 + (size_t)ky * a.nx + x).x : 0.f;
-      acc.x += au * d;
-      acc.y += au;
-    }
-    acc = column_sum2<T>(acc, lds[c], t);
-    if (t == 0) a.wpart[(size_t)b * a.nx + x] = acc;
-  } else {
-    float an = 0.f, kx2 = 0.f;
-    if (a.hmode != 0) {
-      float n = (a.hmode == 1) ? a.n_eff[a.z]
-                               : f_add(a.n0, weighted_dn0(a.wpart + (size_t)b * a.nx, a.nx));
-      an = n_term_f32(n, a.k0);
-      kx2 = kcomp2_f32(x, a.nx, a.dx);
-    }
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int ky = t + m * T;
-      const float2 h = (a.hmode == 0) ? hv[m] : prop_f32(kx2, kcomp2_f32(ky, NY, a.dy), an, a.dz);
-      v[m] = a.conj_h ? c_mulc(v[m], h) : c_mul(v[m], h);
-    }
-    if (a.sw) {
-      // scattering reductions of bpm_3d.py:243-247,335-341 on the k-space
-      // plane after H: sum w |U - delta_{k,0} plain|^2
-      float2 acc = make_float2(0.f, 0.f);
-      const float2 pl = a.pw[a.z];
+    for (int q = 0; q < P; ++q) {
+      const int qx = min(x0 + q, a.nx - x0 - q);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int ky = t + m * T;
-        float2 f = v[m];
-        if (ky == 0 && x == 0) f = c_sub(f, pl);
-        const float af = hypotf(f.x, f.y);
-        const float a2 = af * af;
-        const size_t k = (size_t)ky * a.nx + x;
-        acc.x += a.sw[k] * a2;
-        acc.y += a.gw[k] * a2;
+        hv[q][m] = a.H[(size_t)min(ky, NY - ky) * a.nx + qx];
       }
-      acc = column_sum2<T>(acc, lds[c], t);
-      if (t == 0) a.spart[((size_t)b * a.nz + a.z) * a.nx + x] = acc;
-      __syncthreads();  // lds[c] is the inverse transform's exchange buffer
     }
-    line_fft<NY, E, true, CL>(v, lds[c], t, tw);
+  }
+
+  if constexpr (PHASE != 2) {
+#pragma unroll
+    for (int q = 0; q < P; ++q) line_fft<NY, E, false, CL>(v[q], lds[q * LC + c], t, tw);
   }
 
 #pragma unroll
-  for (int m = 0; m < E; ++m) p[(size_t)(t + m * T) * a.nx] = v[m];
+  for (int q = 0; q < P; ++q) {
+    const int x = x0 + q;
+    float2* line = lds[q * LC + c];
+    if constexpr (PHASE == 1) {
+      const size_t s0 = (size_t)b * a.dn_bstride;
+      float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int ky = t + m * T;
+        const float au = hypotf(v[q][m].x, v[q][m].y);
+        const float d = a.dn ? load_dn(a.dn, a.dn_kind, s0 + (size_t)ky * a.nx + x).x : 0.f;
+        acc.x += au * d;
+        acc.y += au;
+      }
+      acc = column_sum2<T>(acc, line, t);
+      if (t == 0) a.wpart[(size_t)b * a.nx + x] = acc;
+    } else {
+      float an = 0.f, kx2 = 0.f;
+      if (a.hmode != 0) {
+        float n = (a.hmode == 1) ? a.n_eff[a.z]
+                                 : f_add(a.n0, weighted_dn0(a.wpart + (size_t)b * a.nx, a.nx));
+        an = n_term_f32(n, a.k0);
+        kx2 = kcomp2_f32(x, a.nx, a.dx);
+      }
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int ky = t + m * T;
+        const float2 h =
+            (a.hmode == 0) ? hv[q][m] : prop_f32(kx2, kcomp2_f32(ky, NY, a.dy), an, a.dz);
+        v[q][m] = a.conj_h ? c_mulc(v[q][m], h) : c_mul(v[q][m], h);
+      }
+      if (a.sw) {
+        // scattering reductions of bpm_3d.py:243-247,335-341 on the k-space
+        // plane after H: sum w |U - delta_{k,0} plain|^2
+        float2 acc = make_float2(0.f, 0.f);
+        const float2 pl = a.pw[a.z];
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const int ky = t + m * T;
+          float2 f = v[q][m];
+          if (ky == 0 && x == 0) f = c_sub(f, pl);
+          const float af = hypotf(f.x, f.y);
+          const float a2 = af * af;
+          const size_t k = (size_t)ky * a.nx + x;
+          acc.x += a.sw[k] * a2;
+          acc.y += a.gw[k] * a2;
+        }
+        acc = column_sum2<T>(acc, line, t);
+        if (t == 0) a.spart[((size_t)b * a.nz + a.z) * a.nx + x] = acc;
+        __syncthreads();  // the line is the inverse transform's exchange buffer
+      }
+    }
+  }
+  if constexpr (PHASE != 1) {
+#pragma unroll
+    for (int q = 0; q < P; ++q) line_fft<NY, E, true, CL>(v[q], lds[q * LC + c], t, tw);
+  }
+
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    float2* r = p + (size_t)(t + m * T) * a.nx;
+    if constexpr (P == 2)
+      *reinterpret_cast<float4*>(r) = make_float4(v[0][m].x, v[0][m].y, v[1][m].x, v[1][m].y);
+    else
+      *r = v[0][m];
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -755,10 +795,11 @@ static row_fn pick_row(int n, int v) {
 
 template <int PHASE>
 static col_fn pick_col(int n, int v) {
-#define COL_CASE(N)                                                        \
-  case N:                                                                  \
-    return v == 0 ? launch_col<N, 0, PHASE>                                \
-                  : (v == 2 ? launch_col<N, 2, PHASE> : launch_col<N, 1, PHASE>);
+#define COL_CASE(N)                                                              \
+  case N:                                                                        \
+    return v == 0 ? launch_col<N, 0, PHASE>                                      \
+                  : (v == 2 ? launch_col<N, 2, PHASE>                            \
+                            : (v == 3 ? launch_col<N, 3, PHASE> : launch_col<N, 1, PHASE>));
   switch (n) { BPMH_SIZES(COL_CASE) }
 #undef COL_CASE
   return nullptr;
@@ -767,10 +808,19 @@ static col_fn pick_col(int n, int v) {
 static int col_tiles(int nx, int ny, int v) {
 #define TILE_CASE(N)                                                              \
   case N:                                                                         \
-    return nx / (v == 0 ? ColCfg<N, 0>::COLS : (v == 2 ? ColCfg<N, 2>::COLS : ColCfg<N, 1>::COLS));
+    return nx / (v == 0 ? ColCfg<N, 0>::COLS                                      \
+                        : (v == 2 ? ColCfg<N, 2>::COLS                            \
+                                  : (v == 3 ? ColCfg<N, 3>::COLS : ColCfg<N, 1>::COLS)));
   switch (ny) { BPMH_SIZES(TILE_CASE) }
 #undef TILE_CASE
   return 0;
+}
+
+// Column variant 3 loads and stores 16 bytes per column pair: a plane that
+// is only 8-byte aligned (a view at an odd element offset) runs variant 0,
+// the same tiles one column per thread, which gives the same bits.
+static int col_variant_for(int cv, const float2* plane) {
+  return (cv == 3 && (reinterpret_cast<uintptr_t>(plane) & 15)) ? 0 : cv;
 }
 
 static void variant_shape(int nx, int ny, int rv, int cv, int* lines, int* cols) {
@@ -785,8 +835,8 @@ static void variant_shape(int nx, int ny, int rv, int cv, int* lines, int* cols)
 
 extern "C" int bpmh_plan_set_variant(bpmh_plan p, int row_variant, int col_variant) {
   if (!p) return bpmh_set_error(BPMH_ERR_INVALID, "null plan");
-  if (row_variant < 0 || row_variant > 2 || col_variant < 0 || col_variant > 2)
-    return bpmh_set_error(BPMH_ERR_INVALID, "variants are 0, 1 or 2");
+  if (row_variant < 0 || row_variant > 2 || col_variant < 0 || col_variant > 3)
+    return bpmh_set_error(BPMH_ERR_INVALID, "row variants are 0, 1 or 2; column variants 0 to 3");
   p->row_v = row_variant;
   p->col_v = col_variant;
   return BPMH_OK;
@@ -835,9 +885,10 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, 
     return bpmh_set_error(BPMH_ERR_INVALID, "batch too large");
   hipStream_t s = (hipStream_t)stream;
   const row_fn row = pick_row(nx, plan->row_v);
-  const col_fn col0 = pick_col<0>(ny, plan->col_v);
-  const col_fn colA = pick_col<1>(ny, plan->col_v);
-  const col_fn colB = pick_col<2>(ny, plan->col_v);
+  const int cv = col_variant_for(plan->col_v, plane);
+  const col_fn col0 = pick_col<0>(ny, cv);
+  const col_fn colA = pick_col<1>(ny, cv);
+  const col_fn colB = pick_col<2>(ny, cv);
 
   // scratch: weighted per-column partials (batch*nx) + scattering per-column
   // partials (batch*nz*nx)
@@ -1015,7 +1066,7 @@ extern "C" int bpmh_adjoint_segment(bpmh_plan plan, int batch, int z_lo, int z_h
   const int nseg = z_hi - z_lo + 1;
   hipStream_t s = (hipStream_t)stream;
   const row_fn row = pick_row(nx, plan->row_v);
-  const col_fn col0 = pick_col<0>(ny, plan->col_v);
+  const col_fn col0 = pick_col<0>(ny, col_variant_for(plan->col_v, g));
 
   RowArgs ra;
   memset(&ra, 0, sizeof ra);

@@ -53,7 +53,7 @@ def shape_of(plan):
 
 
 def autotune(nx, ny, batch=1, nz=65, reps=3, units=(.1, .1, .1), lam=.5, n0=1.33, save=None):
-    """Time all 3x3 variants for one grid/batch; returns (best, table)."""
+    """Time all 3x4 (row, column) variants for one grid/batch; returns (best, table)."""
     from . import bpm3d as B
     t = _lib.torch()
     plan = B.get_plan(nx, ny, None)
@@ -65,10 +65,12 @@ def autotune(nx, ny, batch=1, nz=65, reps=3, units=(.1, .1, .1), lam=.5, n0=1.33
     plane = t.empty_like(u0)
     table = {}
     for rv in range(3):
-        for cv in range(3):
+        for cv in range(4):
             _lib.call("bpmh_plan_set_variant", plan, rv, cv)
             shp = shape_of(plan)
             key = [shp[2], shp[3]]
+            if cv == 3 and 256 <= ny <= 1024:
+                key.append(2)  # two columns per thread: a kernel of its own
             if key in [v["shape"] for v in table.values()]:
                 continue  # clamped to an identical launch shape
             def run():

@@ -77,10 +77,14 @@ int bpmh_plan_kernel_times(bpmh_plan plan, float* ms, int* counts);
 int bpmh_plan_set_graph(bpmh_plan plan, int enable);
 
 /* Launch-shape variants of the row and column passes (0, 1, 2: half, the
- * default, twice the lines / columns per work-group), chosen per grid size
- * by the autotuner (bpm_amd/tuning.py).  get_variant also reports the lines
- * per row work-group and columns per column tile the variants map to.  No
- * reference counterpart (gputools picks its own FFT launch shapes). */
+ * default, twice the lines / columns per work-group; column variant 3: the
+ * tiles of 0 with two adjacent columns per thread and 16-byte accesses, for
+ * columns of 256-1024 points and 16-byte aligned planes, elsewhere the same
+ * as 0), chosen per grid size and batch by the autotuner
+ * (bpm_amd/tuning.py).  Every variant gives the same bits.  get_variant also
+ * reports the lines per row work-group and columns per column tile the
+ * variants map to.  No reference counterpart (gputools picks its own FFT
+ * launch shapes). */
 int bpmh_plan_set_variant(bpmh_plan plan, int row_variant, int col_variant);
 int bpmh_plan_get_variant(bpmh_plan plan, int* row_variant, int* col_variant,
                           int* lines_per_group, int* cols_per_group);

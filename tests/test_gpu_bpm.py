@@ -295,7 +295,8 @@ def test_served_results_equal_alone_and_in_mix(bpm):
         BpmServer(hbm_budget=1000, slots=1).run(jobs[:1])
 
 
-@pytest.mark.parametrize("nx,ny", [(1024, 1024), (256, 512), (4096, 16), (16, 2048)])
+@pytest.mark.parametrize("nx,ny", [(1024, 1024), (256, 512), (128, 256), (16, 256), (4096, 16),
+                                   (16, 2048)])
 def test_launch_variants_agree(bpm, nx, ny):
     import torch
     from bpm_amd import bpm3d as B
@@ -312,7 +313,7 @@ def test_launch_variants_agree(bpm, nx, ny):
     outs = {}
     try:
         for rv in range(3):
-            for cv in range(3):
+            for cv in range(4):
                 _lib.check(lib.bpmh_plan_set_variant(plan, rv, cv))
                 for mode in ("ignore", "weighted"):
                     p = u0.clone()
@@ -323,6 +324,59 @@ def test_launch_variants_agree(bpm, nx, ny):
         # the launch shape, so every variant gives the same bits
         for (rv, cv, mode), p in outs.items():
             assert torch.equal(p, outs[(1, 1, mode)]), (rv, cv, mode)
+    finally:
+        _lib.check(lib.bpmh_plan_set_variant(plan, 1, 1))
+
+
+@pytest.mark.parametrize("nx,ny", [(512, 512), (64, 1024)])
+def test_paired_column_variant_on_every_column_path(bpm, nx, ny):
+    """Column variant 3 (two adjacent columns per thread, 16-byte accesses)
+    against variant 0 (the same tiles, one column per thread) on the column
+    pass's other paths: the simple mean (propagator formula with n_eff per
+    plane), the scattering sums, and a plane that is only 8-byte aligned (a
+    view at an odd element offset, which the library runs as variant 0).
+    Same bits throughout."""
+    import torch
+    from bpm_amd import bpm3d as B
+    from bpm_amd import _lib
+    say("case paired columns", nx, ny)
+    Nz, nb = 6, 2
+    rng = np.random.default_rng(11)
+    dn = torch.from_numpy(rng.uniform(0, .05, (Nz, ny, nx)).astype(np.float32)).cuda()
+    u0 = torch.from_numpy(np.exp(1j * rng.uniform(0, 3, (nb, ny, nx))).astype(np.complex64)).cuda()
+    units = (.1, .1, .1)
+    H = B.propagator_table(nx, ny, units, LAM, N0, _lib.H_F64)
+    dn0 = dn.mean(dim=(1, 2)).contiguous()
+    n_eff = (dn0 + N0).contiguous()
+    sw = torch.from_numpy(rng.uniform(0, 1, (ny, nx)).astype(np.float32)).cuda()
+    gw = torch.from_numpy(rng.uniform(-1, 1, (ny, nx)).astype(np.float32)).cuda()
+    plain = torch.from_numpy(np.exp(1j * rng.uniform(0, 3, Nz)).astype(np.complex64)).cuda()
+    plan = B.get_plan(nx, ny, None)
+    lib = _lib.load()
+    outs = {}
+    try:
+        for cv in (0, 3):
+            _lib.check(lib.bpmh_plan_set_variant(plan, 1, cv))
+            p = u0.clone()
+            B.propagate_fields(p, Nz, units, LAM, N0, dn=dn, dn_kind=_lib.DN_F32,
+                               mean_mode="simple", n_eff=n_eff, dn0=dn0, tuned=False)
+            outs[(cv, "simple")] = p
+            p = u0.clone()
+            sout = torch.zeros(nb * Nz * 2, dtype=torch.float32, device="cuda")
+            B.propagate_fields(p, Nz, units, LAM, N0, dn=dn, dn_kind=_lib.DN_F32,
+                               mean_mode="ignore", H=H, scat=(sw, gw, plain, sout), tuned=False)
+            outs[(cv, "scat")] = p
+            outs[(cv, "sums")] = sout
+            buf = torch.zeros(nb * ny * nx + 1, dtype=torch.complex64, device="cuda")
+            p = buf[1:].view(nb, ny, nx)
+            assert p.data_ptr() % 16 == 8
+            p.copy_(u0)
+            B.propagate_fields(p, Nz, units, LAM, N0, dn=dn, dn_kind=_lib.DN_F32,
+                               mean_mode="weighted", tuned=False)
+            outs[(cv, "unaligned")] = p.clone()
+        for kind in ("simple", "scat", "sums", "unaligned"):
+            assert torch.equal(outs[(3, kind)], outs[(0, kind)]), kind
+        assert outs[(0, "sums")].abs().sum() > 0
     finally:
         _lib.check(lib.bpmh_plan_set_variant(plan, 1, 1))
 
