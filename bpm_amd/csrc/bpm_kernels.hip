@@ -111,9 +111,10 @@ struct ColCfg : Base<N, col_e(N)> {
                                   ? 1 : (Base<N, col_e(N)>::E <= 4 ? 1024 : 512) / T;
   static constexpr int CMAX = pow2floor(clampi(CMAX0 < TMAX ? CMAX0 : TMAX, 1, 16));
   static constexpr int CMIN = 64 / T < 1 ? 1 : 64 / T;  // >= one full wavefront
-  // V = 3: the tiles of V = 0 with P = 2 adjacent columns per thread, so a
-  // row of a column pair is one 16-byte load and store.  Used for columns
-  // of 256-1024 points (where it was measured); elsewhere it is V = 0.
+  // V = 3: the work-group of V = 0 with P = 2 adjacent columns per thread
+  // (tiles twice as wide), so a row of a column pair is one 16-byte load and
+  // store.  Used for columns of 256-1024 points (where it was measured);
+  // elsewhere it is V = 0.
   static constexpr int P = (V == 3 && N >= 256 && N <= 1024) ? 2 : 1;
   static constexpr int LC = clampi(scale_v(C0, V == 3 ? 0 : V), CMIN, CMAX);  // lane columns
   static constexpr int COLS = LC * P;
@@ -817,8 +818,8 @@ static int col_tiles(int nx, int ny, int v) {
 }
 
 // Column variant 3 loads and stores 16 bytes per column pair: a plane that
-// is only 8-byte aligned (a view at an odd element offset) runs variant 0,
-// the same tiles one column per thread, which gives the same bits.
+// is only 8-byte aligned (a view at an odd element offset) runs variant 0
+// (one column per thread), which gives the same bits.
 static int col_variant_for(int cv, const float2* plane) {
   return (cv == 3 && (reinterpret_cast<uintptr_t>(plane) & 15)) ? 0 : cv;
 }

@@ -78,7 +78,8 @@ int bpmh_plan_set_graph(bpmh_plan plan, int enable);
 
 /* Launch-shape variants of the row and column passes (0, 1, 2: half, the
  * default, twice the lines / columns per work-group; column variant 3: the
- * tiles of 0 with two adjacent columns per thread and 16-byte accesses, for
+ * work-group of 0 with two adjacent columns per thread (tiles twice as wide,
+ * 16-byte accesses), for
  * columns of 256-1024 points and 16-byte aligned planes, elsewhere the same
  * as 0), chosen per grid size and batch by the autotuner
  * (bpm_amd/tuning.py).  Every variant gives the same bits.  get_variant also

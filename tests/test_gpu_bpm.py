@@ -331,7 +331,7 @@ def test_launch_variants_agree(bpm, nx, ny):
 @pytest.mark.parametrize("nx,ny", [(512, 512), (64, 1024)])
 def test_paired_column_variant_on_every_column_path(bpm, nx, ny):
     """Column variant 3 (two adjacent columns per thread, 16-byte accesses)
-    against variant 0 (the same tiles, one column per thread) on the column
+    against variant 0 (the same work-group, one column per thread) on the column
     pass's other paths: the simple mean (propagator formula with n_eff per
     plane), the scattering sums, and a plane that is only 8-byte aligned (a
     view at an odd element offset, which the library runs as variant 0).
