@@ -367,8 +367,10 @@ k_col(ColArgs a) {
   constexpr int P = ColCfg<NY, V>::P, LC = ColCfg<NY, V>::LC;
   // swizzled lines 16/LC float2 apart, or padded lines 9 float2 longer
   // than padded: either way the LC lines a 16-lane group touches together
-  // start in different LDS banks
-  constexpr bool SWZ = LC >= 4;
+  // start in different LDS banks.  Two lane columns swizzle when each
+  // thread holds two columns (1 % faster than padding at 1024^2 in batches)
+  // and pad otherwise (the swizzle was slower there).
+  constexpr bool SWZ = LC >= 4 || (P == 2 && LC >= 2);
   static_assert(COLS <= 16, "at most 16 columns per tile");
   constexpr int LINE = SWZ ? NY + 16 / LC : ColCfg<NY, V>::COL_LINE;
   __shared__ float2 lds[COLS][LINE];
