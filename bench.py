@@ -11,8 +11,8 @@ into the working plane and runs the whole z-loop.
 
 Prints one JSON line.  value = voxel-steps/s = Nx*Ny*(Nz-1) per step / time.
 roofline: the dominant kernel's algorithmic bytes per launch (row pass:
-plane read+write 16 B + dn 4 B + sum-intensity read+write 8 B per voxel;
-column pass: plane read+write 16 B per voxel) divided by its average launch
+plane read+write 16 B + dn 4 B per voxel; column pass: plane read+write
+16 B per voxel; 36 B per voxel-step in all) divided by its average launch
 duration.  That duration comes from two HIP-event measurements on the stream
 the kernels run on: events around the graph-replayed timed region give the
 total, and an eager repeat of the same steps with an event pair around every
@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--no-differentiable", action="store_true")
     ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-floor", action="store_true")
+    ap.add_argument("--no-weighted", action="store_true")
     ap.add_argument("--copy-reference", type=int, default=0, metavar="MIB",
                     help="only run a MIB-sized device copy (counter scale reference)")
     ap.add_argument("--debye-only", action="store_true",
@@ -132,30 +134,43 @@ def debye_extra(torch):
                     "mirror, outputs on device"}
 
 
-def served_extra(seed):
-    """Mixed-size served stream (grids 128^2..1024^2, depth 64..512, 8-64
-    plane-wave angles per job, Poisson arrivals) through the continuous-
-    batching server, and the same jobs through the plain per-request loop."""
+def served_extra(seed, n_jobs=200, load=0.8, plain_jobs=48, runs=2):
+    """Mixed-size served stream (grids 128^2..1024^2, depth U{64..512}, 8-64
+    plane-wave angles per job, spheres as the 256^2 workload) through the
+    continuous-batching server.  Capacity = the n_jobs all queued at t = 0;
+    then `runs` seeded Poisson streams of the same jobs at `load` x that
+    capacity (jobs/s) give latency percentiles and served throughput, with
+    the spread over the runs.  The per-request loop (each angle alone, one
+    job after another) is timed on the first `plain_jobs` jobs; throughputs
+    are compared as voxel-steps/s."""
     from bpm_amd.serving import BpmServer, plain_loop, poisson_jobs
-    jobs = poisson_jobs(seed=seed, n_jobs=24, rate_hz=20.)
+    jobs = poisson_jobs(seed=seed, n_jobs=n_jobs, rate_hz=1.)
     srv = BpmServer(hbm_budget=48 << 30, slots=4)
-    srv.run(jobs[:2], realtime=False)  # warm-up (plans, propagators)
-    r = srv.run(jobs, realtime=True)
-    # capacity: the same jobs all queued at once (no arrival gaps), against
-    # the per-request loop over the same jobs
-    cap = srv.run(jobs, realtime=False)
-    p = plain_loop(jobs)
-    return {"workload": "24 jobs, Poisson 20/s, grids {128,256,512,1024}^2, depth U{64..512}, "
-                        "8-64 angles (tilt <= 0.3 rad), spheres as the 256^2 workload; "
-                        "latency = arrival -> results on host",
-            "p50_ms": r["p50_s"] * 1e3, "p99_ms": r["p99_s"] * 1e3,
-            "served_voxel_steps_per_s": r["throughput"],
-            "served_wall_s": r["wall_s"],
-            "note": "the Poisson run is bounded by the arrivals (the last job arrives "
-                    "about 1.2 s in); capacity = the same jobs all queued at t = 0",
+    try:
+        srv.run(jobs[:4], realtime=False)  # warm-up (plans, propagators)
+        cap = srv.run(jobs, realtime=False)
+        rate = load * n_jobs / cap["wall_s"]
+        rs = []
+        for r_i in range(runs):
+            js = poisson_jobs(seed=seed, n_jobs=n_jobs, rate_hz=rate)
+            rs.append(srv.run(js, realtime=True))
+            print("served run %d: %.1f s, p50 %.1f ms, p99 %.1f ms"
+                  % (r_i, rs[-1]["wall_s"], rs[-1]["p50_s"] * 1e3, rs[-1]["p99_s"] * 1e3),
+                  file=sys.stderr, flush=True)
+    finally:
+        srv.close()
+    p = plain_loop(jobs[:plain_jobs])
+    thr = [r["throughput"] for r in rs]
+    return {"workload": "%d jobs, grids {128,256,512,1024}^2, depth U{64..512}, 8-64 angles "
+                        "(tilt <= 0.3 rad), spheres as the 256^2 workload; 4 slots, 48 GiB "
+                        "HBM budget; latency = arrival -> results on host" % n_jobs,
             "capacity_voxel_steps_per_s": cap["throughput"], "capacity_wall_s": cap["wall_s"],
-            "plain_loop_voxel_steps_per_s": p["throughput"], "plain_wall_s": p["wall_s"],
-            "served_over_plain": r["throughput"] / p["throughput"],
+            "poisson_rate_jobs_per_s": rate, "load_of_capacity": load,
+            "p50_ms": [r["p50_s"] * 1e3 for r in rs], "p99_ms": [r["p99_s"] * 1e3 for r in rs],
+            "served_voxel_steps_per_s": thr, "served_wall_s": [r["wall_s"] for r in rs],
+            "plain_loop_voxel_steps_per_s": p["throughput"], "plain_loop_jobs": plain_jobs,
+            "plain_wall_s": p["wall_s"],
+            "served_over_plain": [v / p["throughput"] for v in thr],
             "capacity_over_plain": cap["throughput"] / p["throughput"]}
 
 
@@ -225,6 +240,7 @@ def small_extra(torch, seed):
         ms = ev0.elapsed_time(ev1) / reps
         vs = nx * ny * (nz - 1) * batch / (ms * 1e-3)
         out["batch%d" % batch] = {"ms_per_propagation": ms, "voxel_steps_per_s": vs,
+                                  "effective_GBps_36B": 36. * vs / 1e9,
                                   "effective_GBps_44B": 44. * vs / 1e9}
     # bpm_3d's default mean_mode "weighted" (k-space weighted dn0 each step:
     # three passes per z-step), batch 1, same outputs
@@ -249,45 +265,43 @@ def small_extra(torch, seed):
     ev1.record()
     torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1) / reps
-    out["weighted_batch1"] = {"ms_per_propagation": ms,
-                              "voxel_steps_per_s": nx * ny * (nz - 1) / (ms * 1e-3)}
+    _lib.check(_lib.load().bpmh_plan_set_graph(B.get_plan(nx, ny), 0))
+    vs = nx * ny * (nz - 1) / (ms * 1e-3)
+    # three passes per step: 52 B per voxel-step
+    out["weighted_batch1"] = {"ms_per_propagation": ms, "voxel_steps_per_s": vs,
+                              "effective_GBps_52B": 52. * vs / 1e9,
+                              "hbm_frac_52B": 52. * vs / 1e9 / HBM_PEAK_GBS}
     return out
 
 
-def batched_extra(torch, seed, dn_g, units, lam, n0):
-    """Batched form at 1024^2: B fields (focused Gaussian beams tilted by
-    seeded plane-wave phases) through the first 129 planes of the benchmark
-    dn (all of them if it has fewer) in one launch sequence per pass,
-    graph-replayed; per-step time is independent of depth."""
+def batched_extra(torch, seed, dn_g, units, lam, n0, batches=(16, 64)):
+    """Batched form at 1024^2: B tilted plane waves through the whole
+    benchmark dn (all its planes) in one launch sequence per pass,
+    graph-replayed; exit fields + summed intensities."""
     from bpm_amd import _lib
     from bpm_amd import bpm3d as B
-    nz = min(129, dn_g.shape[0])
-    ny, nx = dn_g.shape[1], dn_g.shape[2]
+    from bpm_amd.workloads import plane_waves
+    nz, ny, nx = dn_g.shape
     H = B.propagator_table(nx, ny, units, lam, n0, _lib.H_F64)
     rng = np.random.default_rng(seed)
-    out = {"workload": "1024^2, first 129 planes of the benchmark dn, B tilted plane waves "
-                       "(|theta| <= 0.2 rad), exit fields + summed intensities"}
-    for batch in (16, 64):
-        th = rng.uniform(-.2, .2, (batch, 2))
-        k = 2 * np.pi * n0 / lam
-        x = np.arange(nx) * units[0]
-        y = np.arange(ny) * units[1]
-        u0 = np.exp(-1j * k * (np.sin(th[:, 0])[:, None, None] * x[None, None, :] +
-                               np.sin(th[:, 1])[:, None, None] * y[None, :, None])).astype(np.complex64)
-        u0_g = torch.from_numpy(u0).cuda()
+    out = {"workload": "%d^2 x %d planes of the benchmark dn, B tilted plane waves "
+                       "(|theta| <= 0.2 rad), exit fields + summed intensities" % (nx, nz)}
+    for batch in batches:
+        u0_g = torch.from_numpy(plane_waves(nx, ny, units[0], units[1], lam, n0,
+                                            rng.uniform(-.2, .2, (batch, 2)))).cuda()
         plane = torch.empty_like(u0_g)
         acc = torch.empty((batch, ny, nx), dtype=torch.float32, device="cuda")
-        _lib.check(_lib.load().bpmh_plan_set_graph(B.get_plan(nx, ny, batch), 1))
+        plan = B.get_plan(nx, ny, batch)
+        _lib.check(_lib.load().bpmh_plan_set_graph(plan, 1))
 
         def step():
             plane.copy_(u0_g)
             B.propagate_fields(plane, nz, units, lam, n0, dn=dn_g, dn_kind=_lib.DN_F32,
                                mean_mode="ignore", H=H, out_sum=acc)
 
-        for _ in range(2):
-            step()
+        step()
         torch.cuda.synchronize()
-        reps = 3
+        reps = 2
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         ev0.record()
@@ -295,14 +309,98 @@ def batched_extra(torch, seed, dn_g, units, lam, n0):
             step()
         ev1.record()
         torch.cuda.synchronize()
+        _lib.check(_lib.load().bpmh_plan_set_graph(plan, 0))
         ms = ev0.elapsed_time(ev1) / reps
         vs = nx * ny * (nz - 1) * batch / (ms * 1e-3)
-        gbs = 44. * vs / 1e9
-        out["batch%d" % batch] = {"ms_per_propagation": ms, "us_per_field_step": ms * 1e3 / (nz - 1) / batch,
-                                  "voxel_steps_per_s": vs, "effective_GBps_44B": gbs,
-                                  "hbm_frac_44B": gbs / HBM_PEAK_GBS}
+        out["batch%d" % batch] = {"ms_per_propagation": ms,
+                                  "us_per_field_step": ms * 1e3 / (nz - 1) / batch,
+                                  "voxel_steps_per_s": vs,
+                                  "effective_GBps_36B": 36. * vs / 1e9,
+                                  "hbm_frac_36B": 36. * vs / 1e9 / HBM_PEAK_GBS,
+                                  "effective_GBps_44B": 44. * vs / 1e9,
+                                  "hbm_frac_44B": 44. * vs / 1e9 / HBM_PEAK_GBS}
         del plane, acc, u0_g
     return out
+
+
+def weighted_extra(torch, dn_g, u0_g, units, lam, n0, reps=3):
+    """bpm_3d's default mean_mode "weighted" on the benchmark workload (1024^2
+    x all planes of its dn, same focused beam, batch 1), graph-replayed: per
+    step column pass A (FFT_y, k-space partial sums, dn0 formed once by the
+    last work-group), column pass B (H(n0+dn0), IFFT_y) and the row pass,
+    52 B per voxel-step."""
+    from bpm_amd import _lib
+    from bpm_amd import bpm3d as B
+    nz, ny, nx = dn_g.shape
+    plane = torch.empty_like(u0_g)
+    acc = torch.empty((1, ny, nx), dtype=torch.float32, device="cuda")
+    plan = B.get_plan(nx, ny)
+    _lib.check(_lib.load().bpmh_plan_set_graph(plan, 1))
+
+    def step():
+        plane.copy_(u0_g)
+        B.propagate_fields(plane, nz, units, lam, n0, dn=dn_g, dn_kind=_lib.DN_F32,
+                           mean_mode="weighted", out_sum=acc)
+
+    step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        step()
+    ev1.record()
+    torch.cuda.synchronize()
+    _lib.check(_lib.load().bpmh_plan_set_graph(plan, 0))
+    ms = ev0.elapsed_time(ev1) / reps
+    vs = nx * ny * (nz - 1) / (ms * 1e-3)
+    return {"workload": "%d^2 x %d planes, benchmark dn and focused beam, mean_mode weighted "
+                        "(bpm_3d default), exit field + summed intensity, batch 1" % (nx, nz),
+            "ms_per_propagation": ms, "us_per_step": ms * 1e3 / (nz - 1),
+            "voxel_steps_per_s": vs, "effective_GBps_52B": 52. * vs / 1e9,
+            "hbm_frac_52B": 52. * vs / 1e9 / HBM_PEAK_GBS}
+
+
+def floor_extra(torch, dn_g):
+    """Batch-1 floor of a two-pass step: per z-step a pair of element-wise
+    kernels with the passes' algorithmic traffic -- a plane copy (read and
+    write 16 B/voxel) and a plane update with the dn slice (plane read and
+    write + dn read, 20 B/voxel), 36 B per voxel-step -- over all planes of
+    the benchmark dn, captured in one graph and replayed like the z-loop."""
+    nz, ny, nx = dn_g.shape
+    plane = torch.zeros((ny, nx), dtype=torch.complex64, device="cuda")
+    tmp = torch.empty_like(plane)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for z in range(1, 3):  # warm-up outside the capture
+            torch.mul(plane, 1.0, out=tmp)
+            torch.add(tmp, dn_g[z], out=plane)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for z in range(1, nz):
+            torch.mul(plane, 1.0, out=tmp)
+            torch.add(tmp, dn_g[z], out=plane)
+    g.replay()
+    torch.cuda.synchronize()
+    reps = 5
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        g.replay()
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    vs = nx * ny * (nz - 1) / (ms * 1e-3)
+    del g
+    return {"workload": "%d^2 x %d planes, 2 element-wise kernels per step (copy 16 B/voxel, "
+                        "update with dn 20 B/voxel), graph-replayed, batch 1" % (nx, nz),
+            "ms_per_propagation": ms, "us_per_step": ms * 1e3 / (nz - 1),
+            "voxel_steps_per_s": vs, "effective_GBps_36B": 36. * vs / 1e9,
+            "hbm_frac_36B": 36. * vs / 1e9 / HBM_PEAK_GBS}
 
 
 def differentiable_extra(seed):
@@ -313,13 +411,15 @@ def differentiable_extra(seed):
     import torch
     from bpm_amd import adjoint
     from bpm_amd.spheres import spheres_dn_device
-    from bpm_amd.workloads import random_spheres
+    from bpm_amd.workloads import small_workload_spheres
     nz, ny, nx, nb = 256, 512, 512, 16
     units, lam, n0 = (.1, .1, .1), .5, 1.33
     rng = np.random.default_rng(seed)
     size = (nx, ny, nz)
-    dn = spheres_dn_device(size, units, *random_spheres(rng, size, units, 30, (.4, 1.2), (.01, .04)))
-    dn2 = spheres_dn_device(size, units, *random_spheres(rng, size, units, 30, (.4, 1.2), (.01, .04)))
+    # the 256^2 workload's sphere recipe: a centred r = 2 um sphere (dn .05)
+    # plus 20 spheres r ~ U(.5, 1.5) um, dn ~ U(.01, .03)
+    dn = spheres_dn_device(size, units, *small_workload_spheres(rng, size, units))
+    dn2 = spheres_dn_device(size, units, *small_workload_spheres(rng, size, units))
     th = rng.uniform(-.2, .2, (nb, 2))
     k = 2 * np.pi * n0 / lam
     x = np.arange(nx) * units[0]
@@ -347,8 +447,8 @@ def differentiable_extra(seed):
     torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1) / reps
     vsteps = nx * ny * (nz - 1) * nb
-    return {"workload": "512^2 x 256 planes, 16 angles, L2 loss vs second-dn targets, "
-                        "gradient in dn, checkpoint every 16 planes",
+    return {"workload": "512^2 x 256 planes, 16 angles, spheres as the 256^2 workload, L2 loss "
+                        "vs second-dn targets, gradient in dn, checkpoint every 16 planes",
             "ms_forward_backward": ms, "forward_voxel_steps": vsteps,
             "voxel_steps_per_s_forward_equivalent": vsteps / (ms * 1e-3)}
 
@@ -440,13 +540,20 @@ def main():
     cnt = (ctypes.c_int * 4)()
     _lib.check(lib.bpmh_plan_kernel_times(plan, ms, cnt))
     _lib.check(lib.bpmh_plan_set_timing(plan, 0))
+    # the shared plan goes back to eager launches (its captured graphs are
+    # released) before the other workloads run
+    _lib.check(lib.bpmh_plan_set_graph(plan, 0))
 
     ms_step = el / a.steps * 1e3
     vsteps = Nx * Ny * (Nz - 1)
     value = vsteps / (ms_step * 1e-3)
-    # bytes per launch: the row pass at z = 0 has no screen (no dn read) and
-    # starts the sum (no read of it); every other launch reads both
-    row_bytes = (24. + 28. * (Nz - 1)) / Nz * Nx * Ny
+    # algorithmic bytes per launch (36 B per voxel-step: each pass reads and
+    # writes the plane, 16 B per voxel, and the row pass reads dn, 4 B; the
+    # z = 0 row pass has no screen, so no dn read).  The summed-intensity
+    # accumulator (read + write, 8 B per voxel per row pass) is not credited;
+    # the bytes moved with it are reported beside.
+    row_bytes = (16. + 20. * (Nz - 1)) / Nz * Nx * Ny
+    row_bytes_sum = (24. + 28. * (Nz - 1)) / Nz * Nx * Ny
     col_bytes = 16. * Nx * Ny
     ev_total = sum(ms[:4])
     kern = {}
@@ -457,9 +564,12 @@ def main():
         kern[name] = {"avg_us": avg_ms * 1e3, "launches_per_step": launches,
                       "event_avg_us": ms[i] / cnt[i] * 1e3, "share": share,
                       "alg_bytes": nbytes, "GBps": nbytes / (avg_ms * 1e-3) / 1e9}
+    kern["row_pass"]["bytes_with_sum"] = row_bytes_sum
+    kern["row_pass"]["GBps_with_sum"] = row_bytes_sum / (kern["row_pass"]["avg_us"] * 1e-6) / 1e9
     dom = max(kern, key=lambda k: kern[k]["share"])
     ach = kern[dom]["GBps"]
-    step_gbs = (row_bytes + col_bytes) * (Nz - 1) / (ms_step * 1e-3) / 1e9
+    step_gbs = 36. * vsteps / (ms_step * 1e-3) / 1e9
+    step_gbs44 = 44. * vsteps / (ms_step * 1e-3) / 1e9
     traffic = measured_traffic(dom, Nx, Ny, plan_shape(lib, plan, dom))
 
     out = {
@@ -482,6 +592,9 @@ def main():
                    "nx": Nx, "ny": Ny, "nz": Nz, "batch": 1, "seed": a.seed},
         "hbm_frac_step": step_gbs / HBM_PEAK_GBS,
         "step_effective_GBps": step_gbs,
+        "step_bytes_per_voxel_step": 36,
+        "hbm_frac_step_44B": step_gbs44 / HBM_PEAK_GBS,
+        "note_44B": "44 B per voxel-step also counts the summed-intensity read and write",
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                      "traffic": traffic["bytes"] if traffic else None},
@@ -501,6 +614,10 @@ def main():
         out["small_256"] = small_extra(torch, a.seed)
     if not a.no_batched:
         out["batched_1024"] = batched_extra(torch, a.seed, dn_g, units, lam, n0)
+    if not a.no_floor:
+        out["floor_batch1"] = floor_extra(torch, dn_g)
+    if not a.no_weighted:
+        out["weighted_1024"] = weighted_extra(torch, dn_g, u0_g, units, lam, n0)
     if not a.no_cpu_baseline:
         from bpm_amd.workloads import spheres_dn
         planes = min(a.cpu_sample_planes, Nz)

@@ -1,8 +1,10 @@
 """Differentiable split-step BPM: d(loss)/d(dn) through a hand-written adjoint.
 
 ``propagate(dn, u0, units, lam, n0, checkpoint=16)`` returns the exit fields
-of a batch of inputs (e.g. illumination angles) propagated through a real
-dn volume on the static-propagator path of ``Bpm3d`` (no mean correction);
+of a batch of inputs (e.g. illumination angles) propagated through a dn
+volume -- real float32, or complex64 for an absorbing medium (the
+``mult_dn_complex`` screen of bpm/bpmclass/kernels/bpm_3d_kernels.cl, no
+taper) -- on the static-propagator path of ``Bpm3d`` (no mean correction);
 it is a ``torch.autograd.Function``:
 
 * forward: the z-loop runs in ``bpmh_propagate`` segment by segment, keeping
@@ -11,7 +13,9 @@ it is a ``torch.autograd.Function``:
   recomputed from the segment's checkpoint and ``bpmh_adjoint_segment``
   runs the adjoint steps  G_{z-1} = F^-1 conj(H) F (conj(s_z) G_z)  while
   accumulating  dL/ddn_z = k0 dz Im(conj(G_z) u_z)  (G = torch's complex
-  gradient, 2 dL/du*).
+  gradient, 2 dL/du*); for complex dn also dL/dIm dn_z = k0 dz Re(conj(G_z)
+  u_z), returned as the complex gradient dL/dRe + i dL/dIm (torch's
+  convention for complex leaves).
 
 Memory: checkpoints B*ceil(nz/K) planes + one segment of B*(K+1) planes.
 """
@@ -34,6 +38,7 @@ def _make_fn():
         @staticmethod
         def forward(ctx, dn, u0, units, lam, n0, absorbing_width, checkpoint):
             nz, ny, nx = dn.shape
+            kind = _lib.DN_C64 if dn.is_complex() else _lib.DN_F32
             H = B.propagator_table(nx, ny, units, lam, n0, _lib.H_F64)
             plane = u0.detach().to(dtype=t.complex64).contiguous().clone()
             segs = _segments(nz, checkpoint)
@@ -42,32 +47,33 @@ def _make_fn():
             for z0, z1 in segs:
                 ckpts.append(plane.clone())
                 B.propagate_fields(plane, z1 - z0 + 1, units, lam, n0, dn=dnc[z0:],
-                                   dn_kind=_lib.DN_F32, mean_mode="ignore", H=H,
+                                   dn_kind=kind, mean_mode="ignore", H=H,
                                    absorbing_width=absorbing_width)
             ctx.save_for_backward(dnc, *ckpts)
-            ctx.meta = (units, lam, n0, absorbing_width, segs, H)
+            ctx.meta = (units, lam, n0, absorbing_width, segs, H, kind)
             return plane
 
         @staticmethod
         def backward(ctx, grad_out):
             dnc, *ckpts = ctx.saved_tensors
-            units, lam, n0, absorb, segs, H = ctx.meta
+            units, lam, n0, absorb, segs, H, kind = ctx.meta
             nz, ny, nx = dnc.shape
             Bn = grad_out.shape[0]
             g = grad_out.to(dtype=t.complex64).contiguous().clone()
             grad_dn = t.zeros_like(dnc)
-            gitem = t.empty((Bn, ny, nx), dtype=t.float32, device=dnc.device)
+            gitem = t.empty((Bn, ny, nx), dtype=dnc.dtype, device=dnc.device)
             k0dz = float(np.float32(2. * np.pi / lam * units[2]))
             plan = B.get_plan(nx, ny)
             for (z0, z1), ck in zip(reversed(segs), reversed(ckpts)):
                 useg = t.empty((Bn, z1 - z0 + 1, ny, nx), dtype=t.complex64, device=dnc.device)
                 p = ck.clone()
                 B.propagate_fields(p, z1 - z0 + 1, units, lam, n0, dn=dnc[z0:],
-                                   dn_kind=_lib.DN_F32, mean_mode="ignore", H=H,
+                                   dn_kind=kind, mean_mode="ignore", H=H,
                                    absorbing_width=absorb, out_field=useg)
                 _lib.call("bpmh_adjoint_segment", plan, int(Bn), int(z0), int(z1), _lib.ptr(g),
-                          _lib.ptr(useg), _lib.ptr(dnc), _lib.ptr(H), k0dz, int(absorb),
-                          _lib.ptr(grad_dn), _lib.ptr(gitem), _lib.stream_handle())
+                          _lib.ptr(useg), int(useg.numel()), _lib.ptr(dnc), int(kind),
+                          int(dnc.numel()), _lib.ptr(H), k0dz, int(absorb), _lib.ptr(grad_dn),
+                          _lib.ptr(gitem), _lib.stream_handle())
             return grad_dn, g, None, None, None, None, None
 
     return _Fn
@@ -77,13 +83,15 @@ _FN = None
 
 
 def propagate(dn, u0, units=(.1, .1, .1), lam=.5, n0=1., absorbing_width=0, checkpoint=16):
-    """Exit fields (B, Ny, Nx) complex64 of u0 (B, Ny, Nx) through real dn
-    (Nz, Ny, Nx) float32 (both GPU tensors); differentiable in dn and u0."""
+    """Exit fields (B, Ny, Nx) complex64 of u0 (B, Ny, Nx) through dn
+    (Nz, Ny, Nx), float32 or complex64 (both GPU tensors); differentiable in
+    dn and u0."""
     global _FN
     if _FN is None:
         _FN = _make_fn()
-    if dn.is_complex():
-        raise NotImplementedError("the adjoint is implemented for real dn")
+    t = _lib.torch()
+    if dn.dtype not in (t.float32, t.complex64):
+        raise ValueError("dn must be float32 or complex64")
     if dn.dim() != 3 or u0.dim() != 3 or tuple(u0.shape[1:]) != tuple(dn.shape[1:]):
         raise ValueError("dn must be (Nz,Ny,Nx) and u0 (B,Ny,Nx)")
     if checkpoint < 1:

@@ -13,7 +13,7 @@ import numpy as np
 
 from . import _lib
 
-__all__ = ["bpm_3d", "propagate_fields", "get_plan"]
+__all__ = ["bpm_3d", "propagate_fields", "get_plan", "Plan"]
 
 _MEAN = {"ignore": _lib.MEAN_IGNORE, "simple": _lib.MEAN_SIMPLE, "weighted": _lib.MEAN_WEIGHTED}
 _plans = {}
@@ -33,11 +33,45 @@ def get_plan(Nx, Ny, batch=1):
         p = _plans[key] = h
     if batch is None:
         return p
+    apply_tuning(p, Nx, Ny, batch)
+    return p
+
+
+def apply_tuning(plan, Nx, Ny, batch):
+    """Set a plan to the autotuned launch shapes for ``batch`` fields (the
+    cached choice of the nearest tuned batch size), if any are cached."""
     from .tuning import lookup
     v = lookup(int(Nx), int(Ny), int(batch))
     if v is not None:  # two ints on the plan; a replayed graph is keyed by them
-        _lib.call("bpmh_plan_set_variant", p, int(v[0]), int(v[1]))
-    return p
+        _lib.call("bpmh_plan_set_variant", plan, int(v[0]), int(v[1]))
+
+
+class Plan(object):
+    """A plan of its own for an (Ny, Nx) grid on the current device, for
+    callers that propagate on several streams at once (one plan per stream:
+    a plan's scratch and captured graphs serve one stream at a time).
+    Destroyed with ``close()`` or when garbage-collected."""
+
+    def __init__(self, Nx, Ny):
+        _lib.torch()
+        self._h = ctypes.c_void_p()
+        _lib.call("bpmh_plan_create", int(Nx), int(Ny), ctypes.byref(self._h))
+        self.Nx, self.Ny = int(Nx), int(Ny)
+
+    @property
+    def _as_parameter_(self):
+        return self._h
+
+    def close(self):
+        if self._h is not None and self._h.value and _lib._lib is not None:
+            _lib._lib.bpmh_plan_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @atexit.register
@@ -129,7 +163,8 @@ def _check_operands(plane, nz, dn, dn_kind, dn_batch_stride, mm, H, n_eff, dn0, 
 
 def propagate_fields(plane, nz, units, lam, n0=1., dn=None, dn_kind=None, dn_batch_stride=0,
                      mean_mode="ignore", H=None, n_eff=None, dn0=None, absorbing_width=0,
-                     out_field=None, out_intens=None, out_sum=None, scat=None, tuned=True):
+                     out_field=None, out_intens=None, out_sum=None, scat=None, tuned=True,
+                     plan=None):
     """Run nz-1 split-step steps on device for a batch of fields (no host copies).
 
     plane: complex64 tensor (B, Ny, Nx) on the GPU; holds u0 on entry and the
@@ -142,6 +177,9 @@ def propagate_fields(plane, nz, units, lam, n0=1., dn=None, dn_kind=None, dn_bat
     scat:  None or (sw, gw, plain, out) device tensors for the scattering sums.
     tuned: apply the autotuned launch shapes for this batch size (False keeps
            the plan's current ones, as the autotuner needs).
+    plan:  a plan of this grid to run on instead of the shared one of
+           ``get_plan`` (a plan serves one stream at a time: callers that
+           propagate on several streams at once pass a plan per stream).
     """
     B, Ny, Nx = plane.shape
     dx, dy, dz = units
@@ -154,12 +192,19 @@ def propagate_fields(plane, nz, units, lam, n0=1., dn=None, dn_kind=None, dn_bat
         sw, gw, plain, sout = scat
     _check_operands(plane, nz, dn, dn_kind, dn_batch_stride, mm, H, n_eff, dn0,
                     out_field, out_intens, out_sum, sw, gw, plain, sout)
-    _lib.call("bpmh_propagate", get_plan(Nx, Ny, B if tuned else None), int(B), int(nz), _lib.ptr(plane), _lib.ptr(dn),
-              int(dn_kind), int(dn_batch_stride), int(mm), _lib.ptr(H), _lib.ptr(n_eff),
-              _lib.ptr(dn0), float(np.float32(n0)), float(np.float32(k0)), float(np.float32(dx)),
-              float(np.float32(dy)), float(np.float32(dz)), float(np.float32(k0 * dz)),
-              int(absorbing_width), _lib.ptr(out_field), _lib.ptr(out_intens), _lib.ptr(out_sum),
-              _lib.ptr(sw), _lib.ptr(gw), _lib.ptr(plain), _lib.ptr(sout), _lib.stream_handle())
+    if plan is None:
+        plan = get_plan(Nx, Ny, B if tuned else None)
+    elif tuned:
+        apply_tuning(plan, Nx, Ny, B)
+    outs = [o.numel() for o in (out_field, out_intens) if o is not None]
+    _lib.call("bpmh_propagate", plan, int(B), int(nz), _lib.ptr(plane), _lib.ptr(dn),
+              int(dn_kind), int(dn_batch_stride), int(dn.numel() if dn is not None else 0),
+              int(mm), _lib.ptr(H), _lib.ptr(n_eff), _lib.ptr(dn0), float(np.float32(n0)),
+              float(np.float32(k0)), float(np.float32(dx)), float(np.float32(dy)),
+              float(np.float32(dz)), float(np.float32(k0 * dz)), int(absorbing_width),
+              _lib.ptr(out_field), _lib.ptr(out_intens), int(min(outs) if outs else 0),
+              _lib.ptr(out_sum), _lib.ptr(sw), _lib.ptr(gw), _lib.ptr(plain), _lib.ptr(sout),
+              _lib.stream_handle())
     return plane
 
 

@@ -100,9 +100,11 @@ class _Bpm3d_HIP(_Bpm3d_Base):
                           absorbing_width=0, **kwargs):
         """Propagate u0 (plane wave if None) through the whole volume.
 
-        return_full: the (Nz,Ny,Nx) volume, else the last plane;
-        return_intensity: |u|^2 instead of the field (the reference calls an
-        absent kernel ``fill_with_energy`` for this; defined here as |u|^2)."""
+        return_full: the (Nz,Ny,Nx) volume, else the last plane (the complex
+        field, whatever return_intensity says, as _bpm3d_ocl.py:128-131);
+        return_intensity: with return_full, the |u|^2 volume instead of the
+        field (the reference calls an absent kernel ``fill_with_energy`` for
+        it, _bpm3d_ocl.py:88-91,120-123; defined here as |u|^2)."""
         Nx, Ny, Nz = self.size
         if u0 is None:
             u0 = np.ones((Ny, Nx), np.complex64)
@@ -123,8 +125,7 @@ class _Bpm3d_HIP(_Bpm3d_Base):
                          out_field=out_field, out_intens=out_intens)
         if return_full:
             return _lib.to_numpy((out_intens if return_intensity else out_field)[0])
-        u = _lib.to_numpy(plane[0])
-        return np.abs(u) ** 2 if return_intensity else u
+        return _lib.to_numpy(plane[0])
 
     def propagate(self, u0=None, **kwargs):
         """Alias of _propagate_single (bpm/bpm/bpm_class.py:91)."""

@@ -153,15 +153,39 @@ BPMH_DEV float2 prop_f32(float kx2, float ky2, float a, float dz) {
 
 BPMH_DEV float n_term_f32(float n, float k0) { return f_mul(f_mul(f_mul(n, n), k0), k0); }
 
-// dn0 = dn_sum/u_sum from per-tile partials (x = dn part, y = u part).
-// Lane p sums partials p, p+64, ...; the xor butterfly then gives every lane
-// (and every wave of every work-group) the bit-identical total.
-// Every work-group that calls it has a multiple of 64 threads.
-BPMH_DEV float weighted_dn0(const float2* __restrict__ wp, int nparts) {
-  const int lane = threadIdx.x & 63;
+// 8-byte load that reads the device-coherent value (another XCD's store),
+// not a line this XCD's L2 may still hold.
+BPMH_DEV float2 ld_agent(const float2* q) {
+  const unsigned long long u = __hip_atomic_load(
+      reinterpret_cast<unsigned long long*>(const_cast<float2*>(q)), __ATOMIC_RELAXED,
+      __HIP_MEMORY_SCOPE_AGENT);
+  union { unsigned long long u; float2 f; } c;
+  c.u = u;
+  return c.f;
+}
+
+// Weighted mean (bpm_3d.py:149-159,302-303): dn0 = dn_sum/u_sum over the nx
+// per-column partials of item b (x = dn part, y = u part), formed once per
+// step.  Every column-A work-group calls this at its end; its partials were
+// written by lanes of its first wavefront (t == 0), which then releases them
+// and counts the work-group in; the work-group that arrives last (the count
+// reaches ntiles) sums all partials in a fixed order -- lane p sums partials
+// p, p+64, ..., then an xor butterfly -- so dn0 does not depend on which
+// work-group finishes last, stores dn0 for the row pass and column pass B,
+// and resets the counter for the next step.
+BPMH_DEV void weighted_finalize(const float2* __restrict__ wp, int nparts, int ntiles,
+                                unsigned* cnt, float* dn0) {
+  if (threadIdx.x >= 64) return;  // the first wavefront wrote the partials
+  const int lane = threadIdx.x;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  unsigned old = 0;
+  if (lane == 0) old = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  old = __builtin_amdgcn_readfirstlane(old);
+  if (old != (unsigned)(ntiles - 1)) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   float sd = 0.f, su = 0.f;
   for (int p = lane; p < nparts; p += 64) {
-    const float2 v = wp[p];
+    const float2 v = ld_agent(wp + p);
     sd += v.x;
     su += v.y;
   }
@@ -170,7 +194,10 @@ BPMH_DEV float weighted_dn0(const float2* __restrict__ wp, int nparts) {
     sd += __shfl_xor(sd, o, 64);
     su += __shfl_xor(su, o, 64);
   }
-  return sd / su;
+  if (lane == 0) {
+    *dn0 = sd / su;
+    __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // Sum over the T threads of each column of a tile, in a fixed binary-tree
@@ -205,8 +232,7 @@ struct RowArgs {
   int absorb;
   int dn0_mode;            // BPMH_MEAN_*: 0 none, 1 per-plane array, 2 weighted
   const float* dn0_arr;    // simple mode: float32 dn0 per plane
-  const float2* wpart;     // weighted: per-column partials, nparts (= nx) per item
-  int nparts;
+  const float* dn0_item;   // weighted: dn0 of this step per item (column pass A)
   float2* out_field;       // (batch, nz, ny, nx) or null
   float* out_intens;       // (batch, nz, ny, nx) or null
   float* acc;              // (batch, ny, nx) or null: sum_z |u_z|^2
@@ -221,7 +247,7 @@ struct RowArgs {
   const float2* useg;      // forward planes of the segment (batch, nseg, ny, nx)
   long long useg_bstride;  // elements between items in useg
   int useg_z;              // plane of z inside the segment
-  float* gitem;            // (batch, ny, nx) per-item gradient of dn[z]
+  float* gitem;            // (batch, ny, nx) per-item gradient of dn[z] (2 floats per voxel for complex dn)
 };
 
 template <int NX, int V>
@@ -274,7 +300,7 @@ k_row(RowArgs a) {
   if (a.screen) {
     float dn0 = 0.f;
     if (a.dn0_mode == BPMH_MEAN_SIMPLE) dn0 = a.dn0_arr[a.z];
-    else if (a.dn0_mode == BPMH_MEAN_WEIGHTED) dn0 = weighted_dn0(a.wpart + (size_t)b * a.nparts, a.nparts);
+    else if (a.dn0_mode == BPMH_MEAN_WEIGHTED) dn0 = a.dn0_item[b];
     const int disty = min(a.ny - y - 1, y);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -286,7 +312,17 @@ k_row(RowArgs a) {
         float s, c;
         sincos_fast(im, &s, &c);
         const float mag = expf(re);
-        v[m] = c_mul(v[m], make_float2(mag * c, mag * s));
+        if (a.adjoint) {
+          // u = s v with s = e^{k Im dn} e^{-i k Re dn}: dL/dRe dn = k Im(conj(G) u),
+          // dL/dIm dn = k Re(conj(G) u), and G_v = conj(s) G
+          const float2 w = make_float2(v[m].x * ua[m].x + v[m].y * ua[m].y,
+                                       v[m].x * ua[m].y - v[m].y * ua[m].x);
+          reinterpret_cast<float2*>(a.gitem)[((size_t)b * a.ny + y) * NX + x] =
+              make_float2(a.unit_k * w.y, a.unit_k * w.x);
+          v[m] = c_mulc(v[m], make_float2(mag * c, mag * s));
+        } else {
+          v[m] = c_mul(v[m], make_float2(mag * c, mag * s));
+        }
       } else {
         const float ph = f_mul(-a.unit_k, f_sub(d[m].x, dn0));
         float s, c;
@@ -349,6 +385,8 @@ struct ColArgs {
   long long dn_bstride;
   int dn_kind;
   float2* wpart;           // weighted per-column partials (batch, nx)
+  float* dn0_item;         // weighted: dn0 per item, formed at the end of pass A
+  unsigned* wcount;        // weighted: per-item arrival counters of pass A (zero between steps)
   const float* sw;         // scattering weights (ny, nx) or null
   const float* gw;
   const float2* pw;        // plane-wave DC per plane (nz)
@@ -443,8 +481,7 @@ k_col(ColArgs a) {
     } else {
       float an = 0.f, kx2 = 0.f;
       if (a.hmode != 0) {
-        float n = (a.hmode == 1) ? a.n_eff[a.z]
-                                 : f_add(a.n0, weighted_dn0(a.wpart + (size_t)b * a.nx, a.nx));
+        float n = (a.hmode == 1) ? a.n_eff[a.z] : f_add(a.n0, a.dn0_item[b]);
         an = n_term_f32(n, a.k0);
         kx2 = kcomp2_f32(x, a.nx, a.dx);
       }
@@ -490,6 +527,8 @@ k_col(ColArgs a) {
     else
       *r = v[0][m];
   }
+  if constexpr (PHASE == 1)
+    weighted_finalize(a.wpart + (size_t)b * a.nx, a.nx, ntiles, a.wcount + b, a.dn0_item + b);
 }
 
 // ----------------------------------------------------------------------------
@@ -545,9 +584,24 @@ __global__ void k_scatter_final(const float2* spart, float* out, int nz, int npa
 // Kernel kinds timed by the optional HIP-event instrumentation.
 enum { TK_ROW = 0, TK_COL = 1, TK_COL_A = 2, TK_COL_B = 3, TK_N = 4 };
 
+// One captured launch sequence of bpmh_propagate: its argument key, the
+// executable graph, and an event recorded after its latest launch (an
+// executable is destroyed only once that event has completed, so a replay
+// still running on some stream is never destroyed under it).
+struct GraphEntry {
+  std::vector<unsigned char> key;
+  hipGraphExec_t exec;
+  hipEvent_t done;
+  unsigned long long used;  // recency stamp for eviction
+};
+
+// Executables kept per plan (least recently used is evicted).
+constexpr int kGraphCache = 4;
+
 struct bpmh_plan_s {
   int nx, ny;
-  float2* tw_x;
+  std::vector<float2> htw_x, htw_y;  // stage-major twiddles (host)
+  float2* tw_x;                      // their device copies (first propagation)
   float2* tw_y;
   void* scratch;
   size_t scratch_bytes;
@@ -562,9 +616,31 @@ struct bpmh_plan_s {
   int row_v, col_v;             // launch-shape variants (autotuned)
   int graph_mode;
   hipStream_t cap_stream;
-  hipGraphExec_t gexec;
-  std::vector<unsigned char> gkey;
+  std::vector<GraphEntry> graphs;
+  unsigned long long graph_clock;
 };
+
+// Wait for an entry's latest replay, then release it.
+static int graph_release(GraphEntry& g) {
+  hipError_t e1 = hipEventSynchronize(g.done);
+  hipError_t e2 = hipGraphExecDestroy(g.exec);
+  hipError_t e3 = hipEventDestroy(g.done);
+  g.exec = nullptr;
+  g.done = nullptr;
+  hipError_t e = e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3);
+  if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(e));
+  return BPMH_OK;
+}
+
+static int graph_release_all(bpmh_plan p) {
+  int rc = BPMH_OK;
+  for (GraphEntry& g : p->graphs) {
+    const int r = graph_release(g);
+    if (r && !rc) rc = r;
+  }
+  p->graphs.clear();
+  return rc;
+}
 
 static bool pow2_in_range(int n) { return n >= 16 && n <= 4096 && (n & (n - 1)) == 0; }
 
@@ -653,28 +729,38 @@ extern "C" int bpmh_plan_create(int nx, int ny, bpmh_plan* out) {
   p->col_v = 1;
   p->graph_mode = 0;
   p->cap_stream = nullptr;
-  p->gexec = nullptr;
-  (void)hipGetDevice(&p->device);
-  std::vector<float2> tx, ty;
-  make_stage_twiddles(nx, row_e(nx), tx);
-  make_stage_twiddles(ny, col_e(ny), ty);
+  p->graph_clock = 0;
+  p->device = -1;
   p->tw_x = nullptr;
   p->tw_y = nullptr;
-  if (hipMalloc(&p->tw_x, tx.size() * sizeof(float2)) != hipSuccess ||
-      hipMalloc(&p->tw_y, ty.size() * sizeof(float2)) != hipSuccess) {
+  make_stage_twiddles(nx, row_e(nx), p->htw_x);
+  make_stage_twiddles(ny, col_e(ny), p->htw_y);
+  *out = p;
+  return BPMH_OK;
+}
+
+// Device copies of the twiddle tables, made on the plan's first propagation
+// (plan creation itself touches no device); the device current then is the
+// plan's device.
+static int plan_upload(bpmh_plan p) {
+  if (p->tw_x) return BPMH_OK;
+  HIP_TRY(hipGetDevice(&p->device));
+  if (hipMalloc(&p->tw_x, p->htw_x.size() * sizeof(float2)) != hipSuccess ||
+      hipMalloc(&p->tw_y, p->htw_y.size() * sizeof(float2)) != hipSuccess) {
     (void)hipFree(p->tw_x);
     (void)hipFree(p->tw_y);
-    delete p;
+    p->tw_x = p->tw_y = nullptr;
     return bpmh_set_error(BPMH_ERR_NOMEM, "hipMalloc failed for twiddles");
   }
-  if (hipMemcpy(p->tw_x, tx.data(), tx.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(p->tw_y, ty.data(), ty.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMemcpy(p->tw_x, p->htw_x.data(), p->htw_x.size() * sizeof(float2),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p->tw_y, p->htw_y.data(), p->htw_y.size() * sizeof(float2),
+                hipMemcpyHostToDevice) != hipSuccess) {
     (void)hipFree(p->tw_x);
     (void)hipFree(p->tw_y);
-    delete p;
+    p->tw_x = p->tw_y = nullptr;
     return bpmh_set_error(BPMH_ERR_HIP, "hipMemcpy failed for twiddles");
   }
-  *out = p;
   return BPMH_OK;
 }
 
@@ -684,7 +770,7 @@ extern "C" int bpmh_plan_destroy(bpmh_plan p) {
   (void)hipFree(p->tw_y);
   if (p->scratch) (void)hipFree(p->scratch);
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
-  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  (void)graph_release_all(p);
   if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
   delete p;
   return BPMH_OK;
@@ -693,11 +779,7 @@ extern "C" int bpmh_plan_destroy(bpmh_plan p) {
 extern "C" int bpmh_plan_set_graph(bpmh_plan p, int enable) {
   if (!p) return bpmh_set_error(BPMH_ERR_INVALID, "null plan");
   p->graph_mode = enable != 0;
-  if (!p->graph_mode && p->gexec) {
-    HIP_TRY(hipGraphExecDestroy(p->gexec));
-    p->gexec = nullptr;
-    p->gkey.clear();
-  }
+  if (!p->graph_mode) return graph_release_all(p);
   return BPMH_OK;
 }
 
@@ -737,7 +819,16 @@ static int timing_reserve(bpmh_plan p, size_t n) {
   return BPMH_OK;
 }
 
-static int plan_scratch(bpmh_plan p, size_t bytes, void** ptr) {
+__global__ void k_zero_words(unsigned* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+// Plan scratch of at least `bytes`, zero-filled on (re)allocation by a fill
+// kernel on `s` (the weighted-mean arrival counters must start at zero; the
+// passes return them to zero after every step).
+static int plan_scratch(bpmh_plan p, size_t bytes, hipStream_t s, void** ptr) {
   if (bytes > p->scratch_bytes) {
     if (p->scratch) {
       HIP_TRY(hipDeviceSynchronize());
@@ -745,9 +836,12 @@ static int plan_scratch(bpmh_plan p, size_t bytes, void** ptr) {
       p->scratch = nullptr;
       p->scratch_bytes = 0;
     }
+    bytes = (bytes + 255) & ~(size_t)255;
     if (hipMalloc(&p->scratch, bytes) != hipSuccess)
       return bpmh_set_error(BPMH_ERR_NOMEM, "hipMalloc failed for plan scratch");
     p->scratch_bytes = bytes;
+    hipLaunchKernelGGL(k_zero_words, dim3(64), dim3(256), 0, s, (unsigned*)p->scratch, bytes / 4);
+    HIP_TRY(hipGetLastError());
   }
   *ptr = p->scratch;
   return BPMH_OK;
@@ -855,126 +949,188 @@ extern "C" int bpmh_plan_get_variant(bpmh_plan p, int* row_variant, int* col_var
   return BPMH_OK;
 }
 
-extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, const void* dn,
-                              int dn_kind, long long dn_batch_stride, int mean_mode,
-                              const float* H_, const float* n_eff, const float* dn0, float n0,
-                              float k0, float dx, float dy, float dz, float unit_k, int absorb,
-                              float* out_field_, float* out_intens, float* out_sum,
-                              const float* scat_w, const float* g_w, const float* plain_,
-                              float* out_scat, void* stream) {
-  float2* plane = (float2*)plane_;
-  const float2* H = (const float2*)H_;
-  float2* out_field = (float2*)out_field_;
-  const float2* plain = (const float2*)plain_;
+// Validated arguments of one bpmh_propagate call.
+struct PropCall {
+  int batch, nz;
+  float2* plane;
+  const void* dn;
+  int dn_kind;
+  long long dn_bstride, dn_numel;
+  int mean_mode;
+  const float2* H;
+  const float* n_eff;
+  const float* dn0;
+  float n0, k0, dx, dy, dz, unit_k;
+  int absorb;
+  float2* out_field;
+  float* out_intens;
+  long long out_numel;
+  float* out_sum;
+  const float* scat_w;
+  const float* g_w;
+  const float2* plain;
+  float* out_scat;
+};
+
+static int prop_validate(bpmh_plan plan, const PropCall& c) {
+  char b[256];
   if (!plan) return bpmh_set_error(BPMH_ERR_INVALID, "null plan");
-  if (!plane) return bpmh_set_error(BPMH_ERR_INVALID, "null plane");
-  if (batch < 1 || nz < 1) return bpmh_set_error(BPMH_ERR_INVALID, "batch and nz must be >= 1");
-  if (dn_kind < BPMH_DN_NONE || dn_kind > BPMH_DN_C64)
+  if (!c.plane) return bpmh_set_error(BPMH_ERR_INVALID, "null plane");
+  if (c.batch < 1 || c.nz < 1) return bpmh_set_error(BPMH_ERR_INVALID, "batch and nz must be >= 1");
+  if (c.dn_kind < BPMH_DN_NONE || c.dn_kind > BPMH_DN_C64)
     return bpmh_set_error(BPMH_ERR_INVALID, "unknown dn kind");
-  if (dn_kind != BPMH_DN_NONE && !dn) return bpmh_set_error(BPMH_ERR_INVALID, "null dn");
-  if (dn_batch_stride < 0) return bpmh_set_error(BPMH_ERR_INVALID, "dn batch stride must be >= 0");
-  if (mean_mode < BPMH_MEAN_IGNORE || mean_mode > BPMH_MEAN_WEIGHTED)
+  if (c.dn_kind != BPMH_DN_NONE && !c.dn) return bpmh_set_error(BPMH_ERR_INVALID, "null dn");
+  if (c.dn_bstride < 0) return bpmh_set_error(BPMH_ERR_INVALID, "dn batch stride must be >= 0");
+  if (c.mean_mode < BPMH_MEAN_IGNORE || c.mean_mode > BPMH_MEAN_WEIGHTED)
     return bpmh_set_error(BPMH_ERR_INVALID, "unknown mean mode");
-  if (mean_mode == BPMH_MEAN_IGNORE && !H)
+  if (c.mean_mode == BPMH_MEAN_IGNORE && !c.H)
     return bpmh_set_error(BPMH_ERR_INVALID, "mean_mode ignore needs a propagator table");
-  if (mean_mode == BPMH_MEAN_SIMPLE && (!n_eff || !dn0))
+  if (c.mean_mode == BPMH_MEAN_SIMPLE && (!c.n_eff || !c.dn0))
     return bpmh_set_error(BPMH_ERR_INVALID, "mean_mode simple needs n_eff and dn0 arrays");
-  if (absorb < 0) return bpmh_set_error(BPMH_ERR_INVALID, "absorbing width must be >= 0");
-  if (out_scat && (!scat_w || !g_w || !plain))
+  if (c.absorb < 0) return bpmh_set_error(BPMH_ERR_INVALID, "absorbing width must be >= 0");
+  if (c.out_scat && (!c.scat_w || !c.g_w || !c.plain))
     return bpmh_set_error(BPMH_ERR_INVALID, "scattering output needs weights and plane-wave array");
-  const int nx = plan->nx, ny = plan->ny;
   // the column grid's y extent is the batch (at most 65535)
-  if (batch > 65535 || (long long)batch * ny > (1LL << 30))
+  if (c.batch > 65535 || (long long)c.batch * plan->ny > (1LL << 30))
     return bpmh_set_error(BPMH_ERR_INVALID, "batch too large");
-  hipStream_t s = (hipStream_t)stream;
+  const long long n = (long long)plan->nx * plan->ny;
+  if (c.dn_kind != BPMH_DN_NONE) {
+    // planes 0..nz-1 of every item (slice 0 also feeds the weighted mean)
+    const long long need = (long long)(c.batch - 1) * c.dn_bstride + (long long)c.nz * n;
+    if (c.dn_numel < need) {
+      snprintf(b, sizeof b, "dn holds %lld elements; %d planes of %d item(s) at stride %lld read %lld",
+               c.dn_numel, c.nz, c.batch, c.dn_bstride, need);
+      return bpmh_set_error(BPMH_ERR_INVALID, b);
+    }
+  }
+  if (c.out_field || c.out_intens) {
+    const long long need = (long long)c.batch * c.nz * n;
+    if (c.out_numel < need) {
+      snprintf(b, sizeof b, "output volume holds %lld elements; %d item(s) x %d planes write %lld",
+               c.out_numel, c.batch, c.nz, need);
+      return bpmh_set_error(BPMH_ERR_INVALID, b);
+    }
+  }
+  return BPMH_OK;
+}
+
+// Scratch of one call: weighted per-column partials (batch*nx float2),
+// scattering per-column partials (batch*nz*nx float2, only with out_scat),
+// weighted dn0 per item (batch floats) and the weighted arrival counters
+// (batch words, zero between calls).
+struct StepScratch {
+  float2* wpart;
+  float2* spart;
+  float* dn0;
+  unsigned* cnt;
+};
+
+// 0 when the call needs none (no weighted mean, no scattering sums).
+static size_t prop_scratch_bytes(bpmh_plan plan, const PropCall& c) {
+  if (c.mean_mode != BPMH_MEAN_WEIGHTED && !c.out_scat) return 0;
+  const size_t n_w = (size_t)c.batch * plan->nx;
+  const size_t n_s = c.out_scat ? (size_t)c.batch * c.nz * plan->nx : 0;
+  return (n_w + n_s) * sizeof(float2) + 2 * (((size_t)c.batch * 4 + 255) & ~(size_t)255);
+}
+
+static StepScratch prop_scratch_carve(bpmh_plan plan, const PropCall& c, void* base) {
+  StepScratch r;
+  const size_t n_w = (size_t)c.batch * plan->nx;
+  const size_t n_s = c.out_scat ? (size_t)c.batch * c.nz * plan->nx : 0;
+  const size_t seg = ((size_t)c.batch * 4 + 255) & ~(size_t)255;
+  char* p = (char*)base;
+  r.cnt = (unsigned*)p;
+  r.dn0 = (float*)(p + seg);
+  r.wpart = (float2*)(p + 2 * seg);
+  r.spart = r.wpart + n_w;
+  (void)n_s;
+  return r;
+}
+
+// Enqueue (or capture and replay) the whole z-loop of one call on stream s.
+static int prop_run(bpmh_plan plan, const PropCall& c, const StepScratch& sc, hipStream_t s,
+                    bool graph_ok) {
+  const int nx = plan->nx, ny = plan->ny;
+  const int batch = c.batch, nz = c.nz;
   const row_fn row = pick_row(nx, plan->row_v);
-  const int cv = col_variant_for(plan->col_v, plane);
+  const int cv = col_variant_for(plan->col_v, c.plane);
   const col_fn col0 = pick_col<0>(ny, cv);
   const col_fn colA = pick_col<1>(ny, cv);
   const col_fn colB = pick_col<2>(ny, cv);
 
-  // scratch: weighted per-column partials (batch*nx) + scattering per-column
-  // partials (batch*nz*nx)
-  size_t n_w = (size_t)batch * nx;
-  size_t n_s = out_scat ? (size_t)batch * nz * nx : 0;
-  void* scr = nullptr;
-  int rc = plan_scratch(plan, (n_w + n_s + 1) * sizeof(float2), &scr);
-  if (rc) return rc;
-  float2* wpart = (float2*)scr;
-  float2* spart = wpart + n_w;
-
   RowArgs ra;
   memset(&ra, 0, sizeof ra);
-  ra.plane = plane;
+  ra.plane = c.plane;
   ra.tw = plan->tw_x;
-  ra.dn = dn;
-  ra.dn_bstride = dn_batch_stride;
-  ra.dn_kind = dn_kind;
+  ra.dn = c.dn;
+  ra.dn_bstride = c.dn_bstride;
+  ra.dn_kind = c.dn_kind;
   ra.ny = ny;
   ra.nz = nz;
-  ra.unit_k = unit_k;
+  ra.unit_k = c.unit_k;
   ra.inv_n = 1.0f / ((float)nx * (float)ny);
-  ra.absorb = absorb;
-  ra.dn0_mode = (dn_kind == BPMH_DN_NONE) ? 0 : mean_mode;
-  ra.dn0_arr = dn0;
-  ra.wpart = wpart;
-  ra.nparts = nx;
-  ra.out_field = out_field;
-  ra.out_intens = out_intens;
-  ra.acc = out_sum;
+  ra.absorb = c.absorb;
+  ra.dn0_mode = (c.dn_kind == BPMH_DN_NONE) ? 0 : c.mean_mode;
+  ra.dn0_arr = c.dn0;
+  ra.dn0_item = sc.dn0;
+  ra.out_field = c.out_field;
+  ra.out_intens = c.out_intens;
+  ra.acc = c.out_sum;
 
   ColArgs ca;
   memset(&ca, 0, sizeof ca);
-  ca.plane = plane;
+  ca.plane = c.plane;
   ca.tw = plan->tw_y;
   ca.nx = nx;
   ca.nz = nz;
-  ca.hmode = mean_mode;  // 0 table, 1 n_eff array, 2 weighted
-  ca.H = H;
-  ca.n_eff = n_eff;
-  ca.n0 = n0;
-  ca.k0 = k0;
-  ca.dx = dx;
-  ca.dy = dy;
-  ca.dz = dz;
-  ca.dn = dn;
-  ca.dn_bstride = dn_batch_stride;
-  ca.dn_kind = dn_kind;
-  ca.wpart = wpart;
-  ca.sw = out_scat ? scat_w : nullptr;
-  ca.gw = g_w;
-  ca.pw = plain;
-  ca.spart = spart;
+  ca.hmode = c.mean_mode;  // 0 table, 1 n_eff array, 2 weighted
+  ca.H = c.H;
+  ca.n_eff = c.n_eff;
+  ca.n0 = c.n0;
+  ca.k0 = c.k0;
+  ca.dx = c.dx;
+  ca.dy = c.dy;
+  ca.dz = c.dz;
+  ca.dn = c.dn;
+  ca.dn_bstride = c.dn_bstride;
+  ca.dn_kind = c.dn_kind;
+  ca.wpart = sc.wpart;
+  ca.dn0_item = sc.dn0;
+  ca.wcount = sc.cnt;
+  ca.sw = c.out_scat ? c.scat_w : nullptr;
+  ca.gw = c.g_w;
+  ca.pw = c.plain;
+  ca.spart = sc.spart;
 
   if (plan->timing) {
-    rc = timing_reserve(plan, (size_t)nz * 3 + 1);
+    const int rc = timing_reserve(plan, (size_t)nz * 3 + 1);
     if (rc) return rc;
   }
-  const bool use_graph = plan->graph_mode && !plan->timing;
+  const bool use_graph = graph_ok && plan->graph_mode && !plan->timing;
+  const hipStream_t user = s;
   std::vector<unsigned char> key;
   if (use_graph) {
     auto put = [&key](const void* v, size_t n) {
-      const unsigned char* c = (const unsigned char*)v;
-      key.insert(key.end(), c, c + n);
+      const unsigned char* q = (const unsigned char*)v;
+      key.insert(key.end(), q, q + n);
     };
     put(&ra, sizeof ra);
     put(&ca, sizeof ca);
     put(&batch, sizeof batch);
     put(&nz, sizeof nz);
-    put(&mean_mode, sizeof mean_mode);
-    put(&out_scat, sizeof out_scat);
+    put(&c.mean_mode, sizeof c.mean_mode);
+    put(&c.out_scat, sizeof c.out_scat);
     put(&plan->row_v, sizeof plan->row_v);
     put(&plan->col_v, sizeof plan->col_v);
-    if (plan->gexec && key == plan->gkey) {
-      HIP_TRY(hipGraphLaunch(plan->gexec, s));
-      return BPMH_OK;
+    for (GraphEntry& g : plan->graphs) {
+      if (g.key == key) {
+        HIP_TRY(hipGraphLaunch(g.exec, user));
+        HIP_TRY(hipEventRecord(g.done, user));
+        g.used = ++plan->graph_clock;
+        return BPMH_OK;
+      }
     }
     if (!plan->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&plan->cap_stream, hipStreamNonBlocking));
-    if (plan->gexec) {
-      HIP_TRY(hipGraphExecDestroy(plan->gexec));
-      plan->gexec = nullptr;
-      plan->gkey.clear();
-    }
     HIP_TRY(hipStreamBeginCapture(plan->cap_stream, hipStreamCaptureModeThreadLocal));
     s = plan->cap_stream;
   }
@@ -999,10 +1155,10 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, 
   timed(TK_ROW, [&] { row(ra, batch, s); });
   ra.acc_init = 0;
   ra.inverse = 1;
-  ra.screen = dn_kind != BPMH_DN_NONE;
+  ra.screen = c.dn_kind != BPMH_DN_NONE;
   for (int z = 1; z < nz; ++z) {
     ca.z = z;
-    if (mean_mode == BPMH_MEAN_WEIGHTED) {
+    if (c.mean_mode == BPMH_MEAN_WEIGHTED) {
       timed(TK_COL_A, [&] { colA(ca, batch, s); });
       timed(TK_COL_B, [&] { colB(ca, batch, s); });
     } else {
@@ -1012,9 +1168,9 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, 
     ra.forward = z < nz - 1;
     timed(TK_ROW, [&] { row(ra, batch, s); });
   }
-  if (out_scat) {
-    hipLaunchKernelGGL(k_scatter_final, dim3((nz + 63) / 64, batch), dim3(64), 0, s, spart,
-                       out_scat, nz, nx);
+  if (c.out_scat) {
+    hipLaunchKernelGGL(k_scatter_final, dim3((nz + 63) / 64, batch), dim3(64), 0, s, sc.spart,
+                       c.out_scat, nz, nx);
   }
   if (use_graph) {
     hipGraph_t g = nullptr;
@@ -1024,24 +1180,97 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane_, 
       if (g) (void)hipGraphDestroy(g);
       return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(le != hipSuccess ? le : ce));
     }
-    hipError_t ie = hipGraphInstantiate(&plan->gexec, g, nullptr, nullptr, 0);
+    GraphEntry e;
+    e.key = std::move(key);
+    e.exec = nullptr;
+    e.done = nullptr;
+    hipError_t ie = hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    if (ie != hipSuccess) {
-      plan->gexec = nullptr;
-      return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(ie));
+    if (ie != hipSuccess) return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(ie));
+    hipError_t ee = hipEventCreateWithFlags(&e.done, hipEventDisableTiming);
+    if (ee != hipSuccess) {
+      (void)hipGraphExecDestroy(e.exec);
+      return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(ee));
     }
-    plan->gkey = key;
-    HIP_TRY(hipGraphLaunch(plan->gexec, (hipStream_t)stream));
+    if ((int)plan->graphs.size() >= kGraphCache) {
+      size_t lru = 0;
+      for (size_t i = 1; i < plan->graphs.size(); ++i)
+        if (plan->graphs[i].used < plan->graphs[lru].used) lru = i;
+      const int rc = graph_release(plan->graphs[lru]);
+      plan->graphs.erase(plan->graphs.begin() + lru);
+      if (rc) {
+        (void)hipGraphExecDestroy(e.exec);
+        (void)hipEventDestroy(e.done);
+        return rc;
+      }
+    }
+    e.used = ++plan->graph_clock;
+    plan->graphs.push_back(std::move(e));
+    GraphEntry& ne = plan->graphs.back();
+    HIP_TRY(hipGraphLaunch(ne.exec, user));
+    HIP_TRY(hipEventRecord(ne.done, user));
     return BPMH_OK;
   }
   HIP_TRY(hipGetLastError());
   return BPMH_OK;
 }
 
+extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane, const void* dn,
+                              int dn_kind, long long dn_batch_stride, long long dn_numel,
+                              int mean_mode, const float* H, const float* n_eff, const float* dn0,
+                              float n0, float k0, float dx, float dy, float dz, float unit_k,
+                              int absorb, float* out_field, float* out_intens,
+                              long long out_numel, float* out_sum, const float* scat_w,
+                              const float* g_w, const float* plain, float* out_scat,
+                              void* stream) {
+  PropCall c;
+  c.batch = batch;
+  c.nz = nz;
+  c.plane = (float2*)plane;
+  c.dn = dn;
+  c.dn_kind = dn_kind;
+  c.dn_bstride = dn_batch_stride;
+  c.dn_numel = dn_numel;
+  c.mean_mode = mean_mode;
+  c.H = (const float2*)H;
+  c.n_eff = n_eff;
+  c.dn0 = dn0;
+  c.n0 = n0;
+  c.k0 = k0;
+  c.dx = dx;
+  c.dy = dy;
+  c.dz = dz;
+  c.unit_k = unit_k;
+  c.absorb = absorb;
+  c.out_field = (float2*)out_field;
+  c.out_intens = out_intens;
+  c.out_numel = out_numel;
+  c.out_sum = out_sum;
+  c.scat_w = scat_w;
+  c.g_w = g_w;
+  c.plain = (const float2*)plain;
+  c.out_scat = out_scat;
+  int rc = prop_validate(plan, c);
+  if (rc) return rc;
+  rc = plan_upload(plan);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  StepScratch sc = {nullptr, nullptr, nullptr, nullptr};
+  const size_t sb = prop_scratch_bytes(plan, c);
+  if (sb) {
+    void* scr = nullptr;
+    rc = plan_scratch(plan, sb, s, &scr);
+    if (rc) return rc;
+    sc = prop_scratch_carve(plan, c, scr);
+  }
+  return prop_run(plan, c, sc, s, true);
+}
+
 // ----------------------------------------------------------------------------
-// adjoint (gradient of a loss on the last plane with respect to real dn)
+// adjoint (gradient of a loss on the last plane with respect to dn)
 // ----------------------------------------------------------------------------
-// grad[i] += sum_b gitem[b][i] in item order (deterministic)
+// grad[i] += sum_b gitem[b][i] in item order (deterministic); n counts floats
+// (one per voxel for real dn, two for complex dn)
 __global__ void k_grad_reduce(const float* __restrict__ gitem, float* __restrict__ grad,
                               int batch, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1052,7 +1281,8 @@ __global__ void k_grad_reduce(const float* __restrict__ gitem, float* __restrict
 }
 
 extern "C" int bpmh_adjoint_segment(bpmh_plan plan, int batch, int z_lo, int z_hi, float* g_,
-                                    const float* u_seg_, const float* dn, const float* H_,
+                                    const float* u_seg_, long long u_seg_numel, const void* dn,
+                                    int dn_kind, long long dn_numel, const float* H_,
                                     float unit_k, int absorb, float* grad_dn, float* gitem,
                                     void* stream) {
   float2* g = (float2*)g_;
@@ -1060,23 +1290,40 @@ extern "C" int bpmh_adjoint_segment(bpmh_plan plan, int batch, int z_lo, int z_h
   const float2* H = (const float2*)H_;
   if (!plan || !g || !u_seg || !dn || !H || !grad_dn || !gitem)
     return bpmh_set_error(BPMH_ERR_INVALID, "null argument");
+  if (dn_kind != BPMH_DN_F32 && dn_kind != BPMH_DN_C64)
+    return bpmh_set_error(BPMH_ERR_UNSUPPORTED, "the adjoint takes float32 or complex64 dn");
   if (batch < 1 || z_lo < 0 || z_hi < z_lo) return bpmh_set_error(BPMH_ERR_INVALID, "bad range");
   if (batch > 65535 || (long long)batch * plan->ny > (1LL << 30))
     return bpmh_set_error(BPMH_ERR_INVALID, "batch too large");
   if (absorb < 0) return bpmh_set_error(BPMH_ERR_INVALID, "absorbing width must be >= 0");
-  if (z_hi == z_lo) return BPMH_OK;
   const int nx = plan->nx, ny = plan->ny;
   const int nseg = z_hi - z_lo + 1;
+  const long long n2 = (long long)nx * ny;
+  char b[256];
+  if (dn_numel < (long long)(z_hi + 1) * n2) {
+    snprintf(b, sizeof b, "dn and grad_dn hold %lld elements; planes up to %d need %lld",
+             dn_numel, z_hi, (long long)(z_hi + 1) * n2);
+    return bpmh_set_error(BPMH_ERR_INVALID, b);
+  }
+  if (u_seg_numel < (long long)batch * nseg * n2) {
+    snprintf(b, sizeof b, "u_seg holds %lld elements; %d item(s) x %d planes need %lld",
+             u_seg_numel, batch, nseg, (long long)batch * nseg * n2);
+    return bpmh_set_error(BPMH_ERR_INVALID, b);
+  }
+  if (z_hi == z_lo) return BPMH_OK;
+  const int urc = plan_upload(plan);
+  if (urc) return urc;
   hipStream_t s = (hipStream_t)stream;
   const row_fn row = pick_row(nx, plan->row_v);
   const col_fn col0 = pick_col<0>(ny, col_variant_for(plan->col_v, g));
+  const int comps = dn_kind == BPMH_DN_C64 ? 2 : 1;  // floats of gradient per voxel
 
   RowArgs ra;
   memset(&ra, 0, sizeof ra);
   ra.plane = g;
   ra.tw = plan->tw_x;
   ra.dn = dn;
-  ra.dn_kind = BPMH_DN_F32;
+  ra.dn_kind = dn_kind;
   ra.ny = ny;
   ra.nz = z_hi + 1;
   ra.unit_k = unit_k;
@@ -1096,7 +1343,7 @@ extern "C" int bpmh_adjoint_segment(bpmh_plan plan, int batch, int z_lo, int z_h
   ca.H = H;
   ca.conj_h = 1;
 
-  const size_t n = (size_t)nx * ny;
+  const size_t n = (size_t)n2 * comps;
   for (int z = z_hi; z > z_lo; --z) {
     ra.z = z;
     ra.useg_z = z - z_lo;

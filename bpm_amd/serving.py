@@ -6,9 +6,11 @@ summed intensities, copied back to the host.  The server
 
 * admits queued jobs (FIFO) whenever an execution slot is free and the job's
   HBM footprint fits the remaining budget;
-* runs each admitted job on its own HIP stream, its B angles batched into
-  one launch sequence (``bpmh_propagate`` with batch=B sharing dn), so jobs
-  that arrive while others are running overlap with them on the device;
+* runs each admitted job on its slot's HIP stream with the slot's own plan
+  (a plan's scratch and captured graphs serve one stream at a time), its B
+  angles batched into one launch sequence (``bpmh_propagate`` with batch=B
+  sharing dn), so jobs that arrive while others are running overlap with
+  them on the device;
 * retires jobs as their completion events fire, returning their memory to
   the budget and recording latency = host-result-ready - arrival.
 
@@ -25,9 +27,10 @@ import numpy as np
 from . import _lib
 from . import bpm3d as B
 from .spheres import spheres_dn_device
-from .workloads import random_spheres
+from .workloads import plane_waves, small_workload_spheres
 
-__all__ = ["Job", "BpmServer", "poisson_jobs", "plain_loop", "job_footprint"]
+__all__ = ["Job", "BpmServer", "poisson_jobs", "plain_loop", "job_footprint", "job_spheres",
+           "job_u0"]
 
 Job = collections.namedtuple("Job", "jid arrival N nz angles seed")
 
@@ -57,62 +60,72 @@ def job_footprint(job):
 
 
 class _Ctx:
-    """Per-server constants: propagators and plans per grid size."""
+    """Per-server constants: propagators per grid size and a plan per (grid
+    size, slot).  The plans are the server's own (eager launches, their own
+    launch-shape settings), never the shared ones of ``bpm3d.get_plan``."""
 
-    def __init__(self, units, lam, n0, max_batch):
+    def __init__(self, units, lam, n0, slots):
         self.units, self.lam, self.n0 = units, lam, n0
-        self.max_batch = max_batch
+        self.slots = slots
         self.H = {}
+        self.plans = {}
 
     def prepare(self, N):
-        t = _lib.torch()
         if N not in self.H:
             self.H[N] = B.propagator_table(N, N, self.units, self.lam, self.n0, _lib.H_F64)
-            # size the plan scratch for the largest batch once, so no later
-            # call reallocates while other streams are running
-            p = t.zeros((self.max_batch, N, N), dtype=t.complex64, device=_lib.device())
-            B.propagate_fields(p, 2, self.units, self.lam, self.n0, mean_mode="ignore",
-                               H=self.H[N])
-            t.cuda.synchronize()
+            for s in range(self.slots):
+                self.plans[(N, s)] = B.Plan(N, N)
+            _lib.torch().cuda.synchronize()
         return self.H[N]
+
+    def plan(self, N, slot):
+        return self.plans[(N, slot)]
+
+    def close(self):
+        for p in self.plans.values():
+            p.close()
+        self.plans.clear()
+
+
+def job_spheres(job, units):
+    """The job's sphere list (centres, radii, dns), the 256^2 workload's recipe
+    on the job's grid, from the job's seed."""
+    rng = np.random.default_rng(job.seed)
+    return small_workload_spheres(rng, (job.N, job.N, job.nz), units)
+
+
+def job_u0(job, units, lam, n0):
+    """The job's B plane-wave inputs (B, N, N) complex64 on the host."""
+    return plane_waves(job.N, job.N, units[0], units[1], lam, n0, job.angles)
 
 
 def _job_inputs(job, ctx):
     """dn volume (device sphere generator) and the B plane-wave inputs."""
     t = _lib.torch()
-    N, nz = job.N, job.nz
-    dx, dy, dz = ctx.units
-    rng = np.random.default_rng(job.seed)
-    size = (N, N, nz)
-    ext = np.array([N * dx, N * dy, nz * dz])
-    c, r, d = random_spheres(rng, size, ctx.units, 20, (.5, 1.5), (.01, .03))
-    c = np.vstack([ext / 2, c])
-    r = np.concatenate([[2.], r])
-    d = np.concatenate([[.05], d])
-    dn = spheres_dn_device(size, ctx.units, c, r, d)
-    k = 2. * np.pi * ctx.n0 / ctx.lam
-    th = t.from_numpy(np.sin(job.angles).astype(np.float32)).to(_lib.device())
-    x = t.arange(N, device=_lib.device(), dtype=t.float32) * dx
-    y = t.arange(N, device=_lib.device(), dtype=t.float32) * dy
-    ph = -k * (th[:, 0, None, None] * x[None, None, :] + th[:, 1, None, None] * y[None, :, None])
-    u0 = t.polar(t.ones_like(ph), ph)
+    dn = spheres_dn_device((job.N, job.N, job.nz), ctx.units, *job_spheres(job, ctx.units))
+    u0 = t.from_numpy(job_u0(job, ctx.units, ctx.lam, ctx.n0)).to(_lib.device())
     return dn, u0
 
 
 class BpmServer:
-    def __init__(self, hbm_budget, slots=4, units=(.1, .1, .1), lam=.5, n0=1.33, max_batch=64):
+    def __init__(self, hbm_budget, slots=4, units=(.1, .1, .1), lam=.5, n0=1.33):
         self.budget = int(hbm_budget)
         self.slots = int(slots)
-        self.ctx = _Ctx(units, lam, n0, max_batch)
+        self.ctx = _Ctx(units, lam, n0, self.slots)
 
-    def _launch(self, job, stream):
+    def close(self):
+        """Release the server's plans."""
+        self.ctx.close()
+
+    def _launch(self, job, slot, stream):
         t = _lib.torch()
         with t.cuda.stream(stream):
             H = self.ctx.H[job.N]
             dn, plane = _job_inputs(job, self.ctx)
             acc = t.empty((len(job.angles), job.N, job.N), dtype=t.float32, device=_lib.device())
             B.propagate_fields(plane, job.nz, self.ctx.units, self.ctx.lam, self.ctx.n0, dn=dn,
-                               dn_kind=_lib.DN_F32, mean_mode="ignore", H=H, out_sum=acc)
+                               dn_kind=_lib.DN_F32, mean_mode="ignore", H=H, out_sum=acc,
+                               plan=self.ctx.plan(job.N, slot))
             out_u = t.empty(plane.shape, dtype=plane.dtype, pin_memory=True)
             out_i = t.empty(acc.shape, dtype=acc.dtype, pin_memory=True)
             out_u.copy_(plane, non_blocking=True)
@@ -154,7 +167,7 @@ class BpmServer:
                 j = queue.popleft()
                 s = free.pop()
                 used += job_footprint(j)
-                inflight[j.jid] = (j, s, self._launch(j, streams[s]))
+                inflight[j.jid] = (j, s, self._launch(j, s, streams[s]))
             # retire finished jobs
             done = [jid for jid, (_, _, h) in inflight.items() if h["ev"].query()]
             for jid in done:
@@ -186,6 +199,14 @@ def plain_loop(jobs, units=(.1, .1, .1), lam=.5, n0=1.33, keep_results=False):
     ctx = _Ctx(units, lam, n0, 1)
     for N in sorted({j.N for j in jobs}):
         ctx.prepare(N)
+    try:
+        return _plain_loop(jobs, ctx, units, lam, n0, keep_results)
+    finally:
+        ctx.close()
+
+
+def _plain_loop(jobs, ctx, units, lam, n0, keep_results):
+    t = _lib.torch()
     t.cuda.synchronize()
     t0 = time.perf_counter()
     vsteps = 0
@@ -197,7 +218,8 @@ def plain_loop(jobs, units=(.1, .1, .1), lam=.5, n0=1.33, keep_results=False):
             plane = u0[b:b + 1].clone()
             acc = t.empty((1, j.N, j.N), dtype=t.float32, device=_lib.device())
             B.propagate_fields(plane, j.nz, units, lam, n0, dn=dn, dn_kind=_lib.DN_F32,
-                               mean_mode="ignore", H=ctx.H[j.N], out_sum=acc)
+                               mean_mode="ignore", H=ctx.H[j.N], out_sum=acc,
+                               plan=ctx.plan(j.N, 0))
             outs_u.append(plane.cpu())
             outs_i.append(acc.cpu())
         vsteps += j.N * j.N * (j.nz - 1) * len(j.angles)

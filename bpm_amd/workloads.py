@@ -10,8 +10,9 @@
 """
 import numpy as np
 
-__all__ = ["random_spheres", "spheres_dn", "gaussian_beam", "plane_wave",
-           "plain_small_workload", "plain_large_spheres", "plain_large_workload"]
+__all__ = ["random_spheres", "small_workload_spheres", "spheres_dn", "gaussian_beam",
+           "plane_wave", "plane_waves", "plain_small_workload", "plain_large_spheres",
+           "plain_large_workload"]
 
 
 def random_spheres(rng, size, units, n, r_range, dn_range):
@@ -23,6 +24,29 @@ def random_spheres(rng, size, units, n, r_range, dn_range):
     r = rng.uniform(r_range[0], r_range[1], n)
     d = rng.uniform(dn_range[0], dn_range[1], n)
     return c, r, d
+
+
+def small_workload_spheres(rng, size, units):
+    """Spheres of the 256^2 x 256 workload's recipe on any grid: one centred
+    sphere r = 2 um, dn = .05, then 20 spheres with centres uniform in the
+    volume, r ~ U(.5, 1.5) um, dn ~ U(.01, .03) (also the recipe of the
+    served and differentiable workloads); returns (centres, radii, dns)."""
+    Nx, Ny, Nz = size
+    ext = np.array([Nx * units[0], Ny * units[1], Nz * units[2]])
+    c, r, d = random_spheres(rng, size, units, 20, (.5, 1.5), (.01, .03))
+    return (np.vstack([ext / 2, c]), np.concatenate([[2.], r]), np.concatenate([[.05], d]))
+
+
+def plane_waves(Nx, Ny, dx, dy, lam, n0, thetas):
+    """(B, Ny, Nx) complex64 tilted plane waves exp(-i k0 n0 (x sin tx + y sin ty))
+    for thetas (B, 2) = (tx, ty) in radians."""
+    th = np.asarray(thetas, np.float64).reshape(-1, 2)
+    k = 2. * np.pi * n0 / lam
+    x = dx * np.arange(Nx)
+    y = dy * np.arange(Ny)
+    ph = k * (np.sin(th[:, 0])[:, None, None] * x[None, None, :] +
+              np.sin(th[:, 1])[:, None, None] * y[None, :, None])
+    return np.exp(-1.j * ph).astype(np.complex64)
 
 
 def spheres_dn(size, units, centres, radii, dns, dtype=np.float32):
@@ -73,12 +97,7 @@ def plain_small_workload(seed=0, size=(256, 256, 256), units=(.1, .1, .1)):
     """dn and u0 of the 256^2 x 256 plain-BPM workload (plane wave at normal incidence)."""
     rng = np.random.default_rng(seed)
     Nx, Ny, Nz = size
-    ext = np.array([Nx * units[0], Ny * units[1], Nz * units[2]])
-    c, r, d = random_spheres(rng, size, units, 20, (.5, 1.5), (.01, .03))
-    c = np.vstack([ext / 2, c])
-    r = np.concatenate([[2.], r])
-    d = np.concatenate([[.05], d])
-    dn = spheres_dn(size, units, c, r, d)
+    dn = spheres_dn(size, units, *small_workload_spheres(rng, size, units))
     u0 = np.ones((Ny, Nx), np.complex64)
     return u0, dn
 

@@ -7,10 +7,12 @@
  * (re, im) pairs); work is enqueued on
  * the caller's hipStream_t (passed as void*, NULL = default stream) and is
  * asynchronous.  No entry point takes a host array except where stated.
- * The library sees bare pointers and cannot check their extents: every array
- * must hold what the declaration below says the call reads or writes for
- * the given sizes (a short dn volume is read past its end, which faults the
- * device).  bpm_amd.bpm3d.propagate_fields checks this for torch tensors.
+ * The propagation entry points take the element counts of their
+ * variable-length device arrays (the dn volume, the output volumes, the
+ * recomputed forward planes) and return BPMH_ERR_INVALID, before launching
+ * anything, when an array is shorter than the call would read or write.
+ * Arrays whose length follows from the plan and the sizes (planes, H,
+ * per-plane arrays) must hold what the declaration says.
  *
  * The reference binds its device work through gputools/pyopencl
  * (OCLProgram.run_kernel, fft, OCLReductionKernel); each declaration below
@@ -23,7 +25,7 @@
 extern "C" {
 #endif
 
-#define BPMH_VERSION 100
+#define BPMH_VERSION 200
 
 #define BPMH_OK 0
 #define BPMH_ERR_INVALID -1      /* bad argument  (Python: ValueError) */
@@ -71,9 +73,12 @@ int bpmh_plan_kernel_times(bpmh_plan plan, float* ms, int* counts);
 
 /* Graph mode: bpmh_propagate captures its whole launch sequence (2 or 3
  * kernels per z-step) into a HIP graph the first time it sees an argument
- * list and replays the graph (one hipGraphLaunch) while the arguments stay
- * identical.  Replaces the per-kernel Python->pyopencl enqueue of the
- * reference loop (bpm/bpm/bpm_3d.py:283-382).  Ignored while timing is on. */
+ * list and replays it (one hipGraphLaunch) whenever that argument list
+ * recurs.  A plan keeps the 4 most recently used executables; one is
+ * destroyed (on eviction, on bpmh_plan_set_graph(plan, 0) and at
+ * bpmh_plan_destroy) only after its latest replay has completed.  Replaces
+ * the per-kernel Python->pyopencl enqueue of the reference loop
+ * (bpm/bpm/bpm_3d.py:283-382).  Ignored while timing is on. */
 int bpmh_plan_set_graph(bpmh_plan plan, int enable);
 
 /* Launch-shape variants of the row and column passes (0, 1, 2: half, the
@@ -107,6 +112,7 @@ int bpmh_propagator_table(int nx, int ny, int kind, double dx, double dy, double
  *  plane      (batch, ny, nx) c64: u0 on entry, last plane u[nz-1] on exit
  *  dn         (nz, ny, nx) of dn_kind per item, item b at dn + b*dn_batch_stride
  *             elements (0: all items share one volume); NULL for BPMH_DN_NONE
+ *  dn_numel   elements dn holds: at least (batch-1)*dn_batch_stride + nz*ny*nx
  *  mean_mode  BPMH_MEAN_*; IGNORE uses H (a table from bpmh_propagator_table;
  *             only its quadrant j <= ny/2, i <= nx/2 is read, the rest
  *             follows from H[j][i] = H[(ny-j)%ny][(nx-i)%nx]),
@@ -117,17 +123,21 @@ int bpmh_propagator_table(int nx, int ny, int kind, double dx, double dy, double
  *  absorb     taper width in pixels (0: none)
  *  out_field  (batch, nz, ny, nx) c64 or NULL     -- return_full, field
  *  out_intens (batch, nz, ny, nx) f32 or NULL     -- return_full, intensity
+ *  out_numel  elements each non-NULL output volume holds (>= batch*nz*ny*nx)
  *  out_sum    (batch, ny, nx) f32 or NULL: sum over all nz planes of |u|^2
  *  scat_w,g_w (ny, nx) f32 cos(theta), cos(theta)^2; plain (nz) c64 the
  *             Nx*Ny*exp(-i k0 n0 z dz) DC term; out_scat (batch, nz, 2) f32
  *             receives the unnormalised sums (bpm_3d.py:227-260,335-341);
- *             all NULL when scattering is not requested. */
+ *             all NULL when scattering is not requested.
+ * The plan's scratch (weighted-mean and scattering partials) is allocated on
+ * first use and grown when a call needs more; a plan serves one stream at a
+ * time (calls on several streams need a plan each). */
 int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane, const void* dn,
-                   int dn_kind, long long dn_batch_stride, int mean_mode, const float* H,
-                   const float* n_eff, const float* dn0, float n0, float k0, float dx,
-                   float dy, float dz, float unit_k, int absorb, float* out_field,
-                   float* out_intens, float* out_sum, const float* scat_w, const float* g_w,
-                   const float* plain, float* out_scat, void* stream);
+                   int dn_kind, long long dn_batch_stride, long long dn_numel, int mean_mode,
+                   const float* H, const float* n_eff, const float* dn0, float n0, float k0,
+                   float dx, float dy, float dz, float unit_k, int absorb, float* out_field,
+                   float* out_intens, long long out_numel, float* out_sum, const float* scat_w,
+                   const float* g_w, const float* plain, float* out_scat, void* stream);
 
 /* Coated-sphere refractive-index volume (nz, ny, nx) float32: for each voxel
  * at (i dx, j dy, k dz), dmin = distance to the nearest of the n_points
@@ -148,18 +158,23 @@ int bpmh_spheres_dn(int nx, int ny, int nz, double dx, double dy, double dz,
                     const double* spheres, int n_spheres, float* dn, void* stream);
 
 /* Adjoint split-step over planes z_lo..z_hi of a static-propagator
- * (mean_mode ignore / Bpm3d) run with real float32 dn shared by `batch`
- * fields.  g: (batch, ny, nx) c64, on entry G = 2 dL/du*[z_hi] (the
- * gradient convention of torch), on exit G for u[z_lo].  u_seg: the forward
- * planes u[z_lo..z_hi] (batch, z_hi-z_lo+1, ny, nx) c64 (recomputed from a
- * checkpoint by bpmh_propagate with out_field).  dn: the full (nz, ny, nx)
- * volume; grad_dn (nz, ny, nx) f32 receives += dL/ddn for planes
- * z_lo+1..z_hi; gitem: (batch, ny, nx) f32 scratch.  H: the forward table.
+ * (mean_mode ignore / Bpm3d) run with dn shared by `batch` fields, dn_kind
+ * BPMH_DN_F32 (real) or BPMH_DN_C64 (absorbing; no taper, as in the forward
+ * pass).  g: (batch, ny, nx) c64, on entry G = 2 dL/du*[z_hi] (the gradient
+ * convention of torch), on exit G for u[z_lo].  u_seg: the forward planes
+ * u[z_lo..z_hi] (batch, z_hi-z_lo+1, ny, nx) c64 (recomputed from a
+ * checkpoint by bpmh_propagate with out_field), u_seg_numel elements.  dn:
+ * the (nz, ny, nx) volume; grad_dn, of dn's type, receives += dL/ddn for
+ * planes z_lo+1..z_hi (complex dn: dL/dRe + i dL/dIm); both hold dn_numel
+ * elements, at least (z_hi+1)*ny*nx.  gitem: (batch, ny, nx) scratch of dn's
+ * type.  H: the forward table.
  * No reference counterpart (the reference has no gradient; its
- * bpm/bpm/bpm_3d_inverse.py is a non-differentiable phase-ratio experiment). */
+ * bpm/bpm/bpm_3d_inverse.py:83-104 is a non-differentiable phase-ratio
+ * experiment on absorbing dn). */
 int bpmh_adjoint_segment(bpmh_plan plan, int batch, int z_lo, int z_hi, float* g,
-                         const float* u_seg, const float* dn, const float* H, float unit_k,
-                         int absorb, float* grad_dn, float* gitem, void* stream);
+                         const float* u_seg, long long u_seg_numel, const void* dn, int dn_kind,
+                         long long dn_numel, const float* H, float unit_k, int absorb,
+                         float* grad_dn, float* gitem, void* stream);
 
 /* Debye-Wolf kernel variants */
 #define BPMH_DW_INDEX 0  /* debye_wolf, bpm/psf/kernels/psf_debye.cl:24-113 */

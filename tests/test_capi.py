@@ -34,7 +34,7 @@ def test_every_declared_symbol_is_exported_and_bound(lib):
     for n in names:
         assert hasattr(lib, n), n
         assert n in _lib.SIGNATURES, "no ctypes signature for " + n
-    assert lib.bpmh_version() == 100
+    assert lib.bpmh_version() == 200
 
 
 def test_plan_rejects_unsupported_sizes_without_gpu(lib):
@@ -49,8 +49,8 @@ def test_plan_rejects_unsupported_sizes_without_gpu(lib):
 
 def test_argument_validation_paths(lib):
     # invalid before any launch: null plan / bad Debye arguments / bad Bessel order
-    rc = lib.bpmh_propagate(None, 1, 2, None, None, 0, 0, 0, None, None, None, 1., 1., .1, .1,
-                            .1, 1., 0, None, None, None, None, None, None, None, None)
+    rc = lib.bpmh_propagate(None, 1, 2, None, None, 0, 0, 0, 0, None, None, None, 1., 1., .1,
+                            .1, .1, 1., 0, None, None, 0, None, None, None, None, None, None)
     assert rc == _lib.ERR_INVALID
     al = np.array([0., .5, .6], np.float32)
     rc = lib.bpmh_debye_wolf(8, 8, 8, 0, 1, 0, 1, 0, 1, .5, al.ctypes.data, 3, 100, 1., 0., 0,
@@ -110,3 +110,46 @@ def test_propagate_fields_checks_operand_extents_before_launch():
     for kw, nz, msg in cases:
         with pytest.raises(ValueError, match=msg):
             B.propagate_fields(plane, nz, (.1, .1, .1), .5, 1.33, mean_mode="ignore", **kw)
+
+
+def test_library_checks_extents_before_launch(lib):
+    """The library itself rejects a dn volume, output volume or recomputed
+    segment shorter than the call reads or writes (BPMH_ERR_INVALID, nothing
+    launched): the fault class of a 129-step propagation through a 3-plane
+    dn.  Plan creation touches no device, so this runs on the CPU; the
+    pointers are never dereferenced."""
+    h = ctypes.c_void_p()
+    assert lib.bpmh_plan_create(64, 32, ctypes.byref(h)) == 0
+    try:
+        n = 64 * 32
+        fake = ctypes.c_void_p(4096)
+
+        def prop(batch, nz, dn_numel, stride=0, out=None, out_numel=0, kind=_lib.DN_F32):
+            return lib.bpmh_propagate(h, batch, nz, fake, fake, kind, stride, dn_numel,
+                                      _lib.MEAN_IGNORE, fake, None, None, 1.33, 12.5, .1, .1, .1,
+                                      1.25, 0, out, None, out_numel, None, None, None, None,
+                                      None, None)
+
+        cases = [
+            (lambda: prop(1, 129, 3 * n), b"dn holds"),
+            (lambda: prop(2, 5, 9 * n, stride=5 * n), b"dn holds"),  # item 1 ends at 10 planes
+            (lambda: prop(1, 4, 4 * n, out=fake, out_numel=4 * n - 1), b"output volume holds"),
+            (lambda: prop(2, 4, 4 * n, out=fake, out_numel=4 * n), b"output volume holds"),
+        ]
+        for call, msg in cases:
+            rc = call()
+            assert rc == _lib.ERR_INVALID and msg in lib.bpmh_last_error(), (rc, msg)
+        with pytest.raises(ValueError, match="dn holds"):
+            _lib.check(prop(1, 129, 3 * n))
+        # adjoint: dn / grad_dn up to plane z_hi, and the recomputed planes
+        rc = lib.bpmh_adjoint_segment(h, 2, 0, 16, fake, fake, 2 * 17 * n, fake, _lib.DN_F32,
+                                      16 * n, fake, 1.25, 0, fake, fake, None)
+        assert rc == _lib.ERR_INVALID and b"dn and grad_dn hold" in lib.bpmh_last_error()
+        rc = lib.bpmh_adjoint_segment(h, 2, 0, 16, fake, fake, 17 * n, fake, _lib.DN_F32,
+                                      17 * n, fake, 1.25, 0, fake, fake, None)
+        assert rc == _lib.ERR_INVALID and b"u_seg holds" in lib.bpmh_last_error()
+        rc = lib.bpmh_adjoint_segment(h, 1, 0, 4, fake, fake, 5 * n, fake, _lib.DN_F16,
+                                      5 * n, fake, 1.25, 0, fake, fake, None)
+        assert rc == _lib.ERR_UNSUPPORTED
+    finally:
+        assert lib.bpmh_plan_destroy(h) == 0

@@ -16,7 +16,9 @@ def say(*a):
 
 
 def reference_f64(dn, u0, units, lam, n0, absorbing_width=0):
-    """u_z = exp(-i k0 dz dn_z) T * IFFT2(H FFT2(u_{z-1})) in complex128 (torch autograd)."""
+    """u_z = exp(-i k0 dz dn_z) T * IFFT2(H FFT2(u_{z-1})) in complex128 (torch
+    autograd); complex dn (absorbing, Im dn < 0 decays) has no taper, as in
+    the forward pass (bpm/bpmclass/kernels/bpm_3d_kernels.cl mult_dn_complex)."""
     import torch as t
     nz, ny, nx = dn.shape
     dx, dy, dz = units
@@ -26,7 +28,7 @@ def reference_f64(dn, u0, units, lam, n0, absorbing_width=0):
     tmp = n0 ** 2 * k0 ** 2 - kx[None, :] ** 2 - ky[:, None] ** 2
     H = t.from_numpy(np.where(tmp >= 0, np.exp(-1j * dz * np.sqrt(np.maximum(tmp, 0))), 0)).to(dn.device)
     T = None
-    if absorbing_width > 0:
+    if absorbing_width > 0 and not dn.is_complex():
         i, j = np.arange(nx), np.arange(ny)
         d = np.minimum(np.minimum(nx - 1 - i, i)[None, :], np.minimum(ny - 1 - j, j)[:, None])
         T = t.from_numpy(np.where(d < absorbing_width,
@@ -41,13 +43,15 @@ def reference_f64(dn, u0, units, lam, n0, absorbing_width=0):
 
 
 def setup(nz, ny, nx, Bn, seed=0, units=(.1, .1, .1)):
+    """dn and target dn with the 256^2 workload's sphere recipe (a centred
+    r = 2 um sphere plus 20 random ones), and Bn tilted plane waves."""
     import torch as t
     from bpm_amd.spheres import spheres_dn_device
-    from bpm_amd.workloads import random_spheres
+    from bpm_amd.workloads import small_workload_spheres
     rng = np.random.default_rng(seed)
     size = (nx, ny, nz)
-    dn = spheres_dn_device(size, units, *random_spheres(rng, size, units, 30, (.4, 1.2), (.01, .04)))
-    dn2 = spheres_dn_device(size, units, *random_spheres(rng, size, units, 30, (.4, 1.2), (.01, .04)))
+    dn = spheres_dn_device(size, units, *small_workload_spheres(rng, size, units))
+    dn2 = spheres_dn_device(size, units, *small_workload_spheres(rng, size, units))
     th = rng.uniform(-.2, .2, (Bn, 2))
     x = np.arange(nx) * units[0]
     y = np.arange(ny) * units[1]
@@ -83,6 +87,40 @@ def test_adjoint_vs_float64_autograd(absorb):
     assert abs(loss.item() - loss_r.item()) / loss_r.item() < 1e-4
     assert e < 1e-4 and eu < 1e-4
     assert g[0].abs().max().item() == 0  # dn[0] does not enter the propagation
+
+
+def test_adjoint_complex_dn_vs_float64_autograd():
+    """Absorbing (complex) dn: the gradient in Re dn and Im dn (returned as
+    dL/dRe + i dL/dIm, torch's convention for complex leaves) and in u0,
+    against float64 autograd of the same map.  The loss compares with
+    targets through a second, real dn."""
+    import torch as t
+    from bpm_amd import adjoint
+    nz, ny, nx, Bn = 20, 32, 64, 2
+    say("case adjoint complex dn vs f64 autograd")
+    dn, dn2, u0 = setup(nz, ny, nx, Bn, seed=4)
+    dnc = (dn - 0.3j * dn).to(t.complex64).contiguous()
+    target = adjoint.propagate(dn2, u0, lam=LAM, n0=N0).detach()
+    dn_a = dnc.clone().requires_grad_(True)
+    u0_a = u0.clone().requires_grad_(True)
+    out = adjoint.propagate(dn_a, u0_a, lam=LAM, n0=N0, checkpoint=6)
+    loss = (out - target).abs().pow(2).sum()
+    loss.backward()
+    dn_r = dnc.to(t.complex128).clone().requires_grad_(True)
+    u0_r = u0.to(t.complex128).clone().requires_grad_(True)
+    ref = reference_f64(dn_r, u0_r, (.1, .1, .1), LAM, N0)
+    loss_r = (ref - target.to(t.complex128)).abs().pow(2).sum()
+    loss_r.backward()
+    g, gr = dn_a.grad.to(t.complex128), dn_r.grad
+    e = ((g - gr).norm() / gr.norm()).item()
+    e_im = ((g.imag - gr.imag).norm() / gr.imag.norm()).item()
+    eu = ((u0_a.grad.to(t.complex128) - u0_r.grad).norm() / u0_r.grad.norm()).item()
+    say("   loss %.6e vs f64 %.6e; grad_dn rel-L2 %.2e (Im part %.2e); grad_u0 rel-L2 %.2e"
+        % (loss.item(), loss_r.item(), e, e_im, eu))
+    assert dn_a.grad.dtype == t.complex64
+    assert abs(loss.item() - loss_r.item()) / loss_r.item() < 1e-4
+    assert e < 1e-4 and e_im < 1e-4 and eu < 1e-4
+    assert gr.imag.abs().max().item() > 0
 
 
 def test_checkpoint_interval_does_not_change_gradient():

@@ -66,3 +66,62 @@ def test_gaussian_beam_follows_closed_form(bpm):
     say("   rel-L2 vs paraxial closed form at z = %g, %g, %g: %.2e %.2e %.2e"
         % (z0, z0 + Nz // 2 * dz, z0 + (Nz - 1) * dz, *err))
     assert err[0] < 1e-6 and max(err) < 0.02
+
+
+def test_refraction_reference_script_vs_oracle(bpm):
+    """The reference's refraction script (tests/test_refraction.py:6-44): a
+    focused NA 0.3 beam (psf_u0, zfoc at the volume's centre) through an
+    interface tilted by w = 0.8 rad into dn0 = 0.1, 256^3 at 0.1 um, n0 = 1,
+    bpm_3d defaults (weighted mean).  The script only plots; here the whole
+    field volume is checked against the complex64 oracle (bar 1e-4) given
+    the same u0."""
+    Nx = Nz = 256
+    dx, NA, lam, w, dn0 = .1, .3, .5, .8, .1
+    say("case refraction script")
+    x = dx * (np.arange(-Nx / 2, Nx / 2) + .5)
+    z = dx * (np.arange(-Nz / 2, Nz / 2) + .5)
+    Z, Y, X = np.meshgrid(z, x, x, indexing="ij")
+    Zr = np.cos(w) * Z - np.sin(w) * X
+    dn = (dn0 * (Zr > 0)).astype(np.float32)
+    u0 = bpm.psf_u0((Nx, Nx), units=(dx,) * 2, zfoc=dx * (Nz - 1.) / 2., lam=lam, NA=NA)
+    u = bpm.bpm_3d((Nx, Nx, Nz), units=(dx,) * 3, lam=lam, dn=dn, u0=u0)
+    ref = O.bpm_3d((Nx, Nx, Nz), (dx,) * 3, lam, u0, dn, 1.)
+    e = O.rel_l2(u, ref)
+    say("   field volume rel-L2 vs c64 oracle %.2e" % e)
+    assert u.shape == (Nz, Nx, Nx) and e < 1e-4
+
+
+def test_refraction_follows_snell(bpm):
+    """A Gaussian beam (w0 = 3 um) at normal incidence crosses a plane
+    interface tilted by w = 0.4 rad into n2 = n0 + dn0 (n0 = 1.33, dn0 =
+    0.05).  Behind the interface its centroid moves on a straight line whose
+    slope dx/dz is measured over 25 um.  Phase matching of the split step
+    (thin phase screen, propagator in n0) gives the transverse wavenumber
+    kx = -k0 dn0 tan w, so dx/dz = kx / sqrt(n0^2 k0^2 - kx^2): checked to
+    1 %.  Snell's law, n0 sin w = n2 sin(w - delta), bends the beam towards
+    the normal (dx/dz < 0) by delta; the split step agrees with it to first
+    order in dn0 (their ratio is n2/n0 = 1.038 here): checked to 5 %."""
+    from bpm_amd.workloads import gaussian_beam
+    Nx, Nz = 256, 512
+    dx, lam, n0, dn0, w = .1, .5, 1.33, .05, .4
+    say("case Snell")
+    x = dx * (np.arange(Nx) - Nx / 2)
+    z = dx * np.arange(Nz)
+    Z, Y, X = np.meshgrid(z, x, x, indexing="ij")
+    dn = (dn0 * (np.cos(w) * (Z - 10.) - np.sin(w) * X > 0)).astype(np.float32)
+    u0 = gaussian_beam(Nx, Nx, dx, dx, lam, n0, 3., 0.)
+    I = bpm.bpm_3d((Nx, Nx, Nz), (dx,) * 3, lam=lam, u0=u0, dn=dn, n0=n0, mean_mode="ignore",
+                   return_field=False, absorbing_width=16)
+    ks = np.arange(250, Nz)  # z from 25 um on: behind the interface
+    prof = I[ks].sum(axis=1).astype(np.float64)  # (planes, x)
+    xc = (prof * x[None, :]).sum(1) / prof.sum(1)
+    slope = np.polyfit(z[ks], xc, 1)[0]
+    k0 = 2 * np.pi / lam
+    kx = -k0 * dn0 * np.tan(w)
+    pred = kx / np.sqrt((n0 * k0) ** 2 - kx ** 2)
+    delta = w - np.arcsin(n0 * np.sin(w) / (n0 + dn0))
+    say("   centroid slope %.5f; split-step phase matching %.5f; Snell %.5f"
+        % (slope, pred, -np.tan(delta)))
+    assert slope < 0
+    assert abs(slope / pred - 1) < .01
+    assert abs(slope / -np.tan(delta) - 1) < .05
