@@ -37,7 +37,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 (matrix, and vector with packed math)
 
 
 def parse():
@@ -111,7 +111,11 @@ def cpu_baseline(u0, dn, units, lam, n0, planes, planes_1core):
 
 
 def debye_extra(torch):
-    """The 256^3 Debye-Wolf workload (NA 1.1, n0 1.33, 512 theta steps)."""
+    """The 256^3 Debye-Wolf workload (NA 1.1, n0 1.33, 512 theta steps): the
+    whole call (octant quadrature written straight to the full grid), then
+    one call with HIP events around each of its kernels for the product's
+    MFMA rate."""
+    import ctypes
     from bpm_amd import focus_fields as Q
     from bpm_amd import _lib
     shape, units, lam, NA, n0, S = (256, 256, 256), (.05,) * 3, .5, 1.1, 1.33, 512
@@ -123,15 +127,33 @@ def debye_extra(torch):
         Q.focus_field_debye_device(shape, units, lam, NA, n0, S)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / reps
+    lib = _lib.load()
+    _lib.check(lib.bpmh_debye_set_timing(1))
+    Q.focus_field_debye_device(shape, units, lam, NA, n0, S)
+    ms = (ctypes.c_float * 4)()
+    cnt = (ctypes.c_int * 4)()
+    _lib.check(lib.bpmh_debye_kernel_times(ms, cnt))
+    _lib.check(lib.bpmh_debye_set_timing(0))
+    # the product: for every (plane, table column, sample) 6 real
+    # multiply-adds (Re and Im of I0, I1, I2), 12 flop; the table runs over
+    # the 129*130/2 column pairs of the 129^2 octant plane and is padded to
+    # whole MFMA tiles (planes to 144, columns to 8416, samples to 528)
+    nz, npairs, total = 129, 129 * 130 // 2, S + 1
+    zp, cp, tp = 144, (npairs + 31) // 32 * 32, (total + 15) // 16 * 16
+    prod_s = ms[3] * 1e-3
+    mfma_tf = 12. * zp * cp * tp / prod_s / 1e12
+    useful_tf = 12. * nz * npairs * total / prod_s / 1e12
     samples = 129 ** 3 * (S + 1)
-    # 6 fused multiply-adds (12 flop) per theta-sample and voxel in the
-    # product kernel; the per-sample Bessel and phase tables are not counted
-    tflops = samples * 12 / dt / 1e12
     return {"workload": "Debye-Wolf 256^3, NA 1.1, n0 1.33, 512 theta steps, x-polarised",
             "ms": dt * 1e3, "theta_samples_per_s": samples / dt,
-            "tflops_12_per_sample": tflops, "frac_fp32_vector_peak": tflops / FP32_PEAK_TFLOPS,
-            "note": "octant quadrature (sample/phase tables + tiled product) + full-grid "
-                    "mirror, outputs on device"}
+            "kernel_us": {"params": ms[0] * 1e3, "sample_table": ms[1] * 1e3,
+                          "phase_table": ms[2] * 1e3, "product_and_outputs": ms[3] * 1e3},
+            "product_mfma_tflops": mfma_tf, "product_mfma_frac": mfma_tf / FP32_PEAK_TFLOPS,
+            "product_useful_tflops": useful_tf,
+            "note": "MFMA fraction = flops the MFMA units execute over the padded "
+                    "144 x 8416 x 528 product (12 per plane, column pair and sample) / the "
+                    "product kernel's time (which includes writing the 470 MB of outputs) / "
+                    "157.3 TFLOP/s dense FP32 matrix peak; useful = unpadded 129 x 8385 x 513"}
 
 
 def served_extra(seed, n_jobs=200, load=0.8, plain_jobs=48, runs=2):
@@ -362,45 +384,56 @@ def weighted_extra(torch, dn_g, u0_g, units, lam, n0, reps=3):
 
 
 def floor_extra(torch, dn_g):
-    """Batch-1 floor of a two-pass step: per z-step a pair of element-wise
-    kernels with the passes' algorithmic traffic -- a plane copy (read and
-    write 16 B/voxel) and a plane update with the dn slice (plane read and
-    write + dn read, 20 B/voxel), 36 B per voxel-step -- over all planes of
-    the benchmark dn, captured in one graph and replayed like the z-loop."""
+    """Batch-1 floors of a two-pass step over all planes of the benchmark dn,
+    each captured in one graph and replayed like the z-loop:
+    * "copy_pair": two plane copies per step (plane -> tmp -> plane, device
+      copies), 32 B per voxel-step;
+    * "elementwise_pair": a plane copy (16 B/voxel) and a plane update with
+      the dn slice (plane read and write + dn read, 20 B/voxel), 36 B per
+      voxel-step, as torch element-wise kernels."""
     nz, ny, nx = dn_g.shape
     plane = torch.zeros((ny, nx), dtype=torch.complex64, device="cuda")
     tmp = torch.empty_like(plane)
-    side = torch.cuda.Stream()
-    side.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(side):
-        for z in range(1, 3):  # warm-up outside the capture
-            torch.mul(plane, 1.0, out=tmp)
-            torch.add(tmp, dn_g[z], out=plane)
-    torch.cuda.current_stream().wait_stream(side)
-    torch.cuda.synchronize()
-    g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g):
-        for z in range(1, nz):
-            torch.mul(plane, 1.0, out=tmp)
-            torch.add(tmp, dn_g[z], out=plane)
-    g.replay()
-    torch.cuda.synchronize()
-    reps = 5
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    for _ in range(reps):
+
+    def copy_pair(z):
+        tmp.copy_(plane)
+        plane.copy_(tmp)
+
+    def elementwise_pair(z):
+        torch.mul(plane, 1.0, out=tmp)
+        torch.add(tmp, dn_g[z], out=plane)
+
+    out = {"workload": "%d^2 x %d planes, batch 1, graph-replayed" % (nx, nz)}
+    for name, fn, nbytes in (("copy_pair", copy_pair, 32.), ("elementwise_pair", elementwise_pair, 36.)):
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for z in range(1, 3):  # warm-up outside the capture
+                fn(z)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for z in range(1, nz):
+                fn(z)
         g.replay()
-    ev1.record()
-    torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1) / reps
-    vs = nx * ny * (nz - 1) / (ms * 1e-3)
-    del g
-    return {"workload": "%d^2 x %d planes, 2 element-wise kernels per step (copy 16 B/voxel, "
-                        "update with dn 20 B/voxel), graph-replayed, batch 1" % (nx, nz),
-            "ms_per_propagation": ms, "us_per_step": ms * 1e3 / (nz - 1),
-            "voxel_steps_per_s": vs, "effective_GBps_36B": 36. * vs / 1e9,
-            "hbm_frac_36B": 36. * vs / 1e9 / HBM_PEAK_GBS}
+        torch.cuda.synchronize()
+        reps = 5
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(reps):
+            g.replay()
+        ev1.record()
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / reps
+        vs = nx * ny * (nz - 1) / (ms * 1e-3)
+        del g
+        out[name] = {"bytes_per_voxel_step": nbytes, "ms_per_propagation": ms,
+                     "us_per_step": ms * 1e3 / (nz - 1), "voxel_steps_per_s": vs,
+                     "effective_GBps": nbytes * vs / 1e9,
+                     "hbm_frac": nbytes * vs / 1e9 / HBM_PEAK_GBS}
+    return out
 
 
 def differentiable_extra(seed):

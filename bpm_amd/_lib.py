@@ -46,6 +46,9 @@ SIGNATURES = {
     "bpmh_debye_wolf_points": (_I, [_I, _P, _P, _P, _F, _P, _I, _I, _F, _F, _I, _F, _I, _P,
                                     _P, _P, _P, _P]),
     "bpmh_debye_mirror": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "bpmh_focus_field_debye": (_I, [_I, _I, _I, _D, _D, _D, _F, _P, _I, _I, _P, _P, _P, _P, _P]),
+    "bpmh_debye_set_timing": (_I, [_I]),
+    "bpmh_debye_kernel_times": (_I, [_P, _P]),
     "bpmh_bessel_fill": (_I, [_I, _P, _P, _I, _P]),
     "bpmh_cylindrical": (_I, [_I, _I, _F, _F, _F, _F, _F, _F, _I, _I, _P, _P, _P, _P, _P]),
     "bpmh_fill_dn_coated": (_I, [_I, _I, _I, _F, _F, _F, _P, _I, _F, _F, _F, _F, _P, _P]),

@@ -153,51 +153,32 @@ BPMH_DEV float2 prop_f32(float kx2, float ky2, float a, float dz) {
 
 BPMH_DEV float n_term_f32(float n, float k0) { return f_mul(f_mul(f_mul(n, n), k0), k0); }
 
-// 8-byte load that reads the device-coherent value (another XCD's store),
-// not a line this XCD's L2 may still hold.
-BPMH_DEV float2 ld_agent(const float2* q) {
-  const unsigned long long u = __hip_atomic_load(
-      reinterpret_cast<unsigned long long*>(const_cast<float2*>(q)), __ATOMIC_RELAXED,
-      __HIP_MEMORY_SCOPE_AGENT);
-  union { unsigned long long u; float2 f; } c;
-  c.u = u;
-  return c.f;
-}
-
 // Weighted mean (bpm_3d.py:149-159,302-303): dn0 = dn_sum/u_sum over the nx
-// per-column partials of item b (x = dn part, y = u part), formed once per
-// step.  Every column-A work-group calls this at its end; its partials were
-// written by lanes of its first wavefront (t == 0), which then releases them
-// and counts the work-group in; the work-group that arrives last (the count
-// reaches ntiles) sums all partials in a fixed order -- lane p sums partials
-// p, p+64, ..., then an xor butterfly -- so dn0 does not depend on which
-// work-group finishes last, stores dn0 for the row pass and column pass B,
-// and resets the counter for the next step.
-BPMH_DEV void weighted_finalize(const float2* __restrict__ wp, int nparts, int ntiles,
-                                unsigned* cnt, float* dn0) {
-  if (threadIdx.x >= 64) return;  // the first wavefront wrote the partials
-  const int lane = threadIdx.x;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  unsigned old = 0;
-  if (lane == 0) old = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  old = __builtin_amdgcn_readfirstlane(old);
-  if (old != (unsigned)(ntiles - 1)) return;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  float sd = 0.f, su = 0.f;
-  for (int p = lane; p < nparts; p += 64) {
-    const float2 v = ld_agent(wp + p);
-    sd += v.x;
-    su += v.y;
-  }
+// per-column partials of item b (x = dn part, y = u part) that column pass A
+// wrote.  Column pass B forms it once per work-group: lane p of the first
+// wavefront sums partials p, p+64, ... and an xor butterfly completes the
+// sum (a fixed order, so every work-group gets the same bits); the result is
+// broadcast through LDS, and the work-group of tile 0 stores it for the row
+// pass.  The kernel boundary after pass A orders the partials' stores before
+// these loads.
+BPMH_DEV float weighted_dn0_group(const float2* __restrict__ wp, int nparts, float* slot) {
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    float sd = 0.f, su = 0.f;
+    for (int p = lane; p < nparts; p += 64) {
+      const float2 v = wp[p];
+      sd += v.x;
+      su += v.y;
+    }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    sd += __shfl_xor(sd, o, 64);
-    su += __shfl_xor(su, o, 64);
+    for (int o = 32; o > 0; o >>= 1) {
+      sd += __shfl_xor(sd, o, 64);
+      su += __shfl_xor(su, o, 64);
+    }
+    if (lane == 0) *slot = sd / su;
   }
-  if (lane == 0) {
-    *dn0 = sd / su;
-    __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  __syncthreads();
+  return *slot;
 }
 
 // Sum over the T threads of each column of a tile, in a fixed binary-tree
@@ -385,8 +366,7 @@ struct ColArgs {
   long long dn_bstride;
   int dn_kind;
   float2* wpart;           // weighted per-column partials (batch, nx)
-  float* dn0_item;         // weighted: dn0 per item, formed at the end of pass A
-  unsigned* wcount;        // weighted: per-item arrival counters of pass A (zero between steps)
+  float* dn0_item;         // weighted: dn0 per item, stored by pass B for the row pass
   const float* sw;         // scattering weights (ny, nx) or null
   const float* gw;
   const float2* pw;        // plane-wave DC per plane (nz)
@@ -456,6 +436,13 @@ k_col(ColArgs a) {
     }
   }
 
+  float dn0w = 0.f;
+  if constexpr (PHASE == 2) {
+    __shared__ float dn0_slot;
+    dn0w = weighted_dn0_group(a.wpart + (size_t)b * a.nx, a.nx, &dn0_slot);
+    if (bx == 0 && threadIdx.x == 0) a.dn0_item[b] = dn0w;
+  }
+
   if constexpr (PHASE != 2) {
 #pragma unroll
     for (int q = 0; q < P; ++q) line_fft<NY, E, false, CL>(v[q], lds[q * LC + c], t, tw);
@@ -481,7 +468,7 @@ k_col(ColArgs a) {
     } else {
       float an = 0.f, kx2 = 0.f;
       if (a.hmode != 0) {
-        float n = (a.hmode == 1) ? a.n_eff[a.z] : f_add(a.n0, a.dn0_item[b]);
+        float n = (a.hmode == 1) ? a.n_eff[a.z] : f_add(a.n0, dn0w);
         an = n_term_f32(n, a.k0);
         kx2 = kcomp2_f32(x, a.nx, a.dx);
       }
@@ -527,8 +514,6 @@ k_col(ColArgs a) {
     else
       *r = v[0][m];
   }
-  if constexpr (PHASE == 1)
-    weighted_finalize(a.wpart + (size_t)b * a.nx, a.nx, ntiles, a.wcount + b, a.dn0_item + b);
 }
 
 // ----------------------------------------------------------------------------
@@ -819,16 +804,8 @@ static int timing_reserve(bpmh_plan p, size_t n) {
   return BPMH_OK;
 }
 
-__global__ void k_zero_words(unsigned* p, size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    p[i] = 0u;
-}
-
-// Plan scratch of at least `bytes`, zero-filled on (re)allocation by a fill
-// kernel on `s` (the weighted-mean arrival counters must start at zero; the
-// passes return them to zero after every step).
-static int plan_scratch(bpmh_plan p, size_t bytes, hipStream_t s, void** ptr) {
+// Plan scratch of at least `bytes` (grown, never shrunk).
+static int plan_scratch(bpmh_plan p, size_t bytes, void** ptr) {
   if (bytes > p->scratch_bytes) {
     if (p->scratch) {
       HIP_TRY(hipDeviceSynchronize());
@@ -840,8 +817,6 @@ static int plan_scratch(bpmh_plan p, size_t bytes, hipStream_t s, void** ptr) {
     if (hipMalloc(&p->scratch, bytes) != hipSuccess)
       return bpmh_set_error(BPMH_ERR_NOMEM, "hipMalloc failed for plan scratch");
     p->scratch_bytes = bytes;
-    hipLaunchKernelGGL(k_zero_words, dim3(64), dim3(256), 0, s, (unsigned*)p->scratch, bytes / 4);
-    HIP_TRY(hipGetLastError());
   }
   *ptr = p->scratch;
   return BPMH_OK;
@@ -1014,15 +989,13 @@ static int prop_validate(bpmh_plan plan, const PropCall& c) {
   return BPMH_OK;
 }
 
-// Scratch of one call: weighted per-column partials (batch*nx float2),
-// scattering per-column partials (batch*nz*nx float2, only with out_scat),
-// weighted dn0 per item (batch floats) and the weighted arrival counters
-// (batch words, zero between calls).
+// Scratch of one call: weighted dn0 per item (batch floats), weighted
+// per-column partials (batch*nx float2) and scattering per-column partials
+// (batch*nz*nx float2, only with out_scat).
 struct StepScratch {
   float2* wpart;
   float2* spart;
   float* dn0;
-  unsigned* cnt;
 };
 
 // 0 when the call needs none (no weighted mean, no scattering sums).
@@ -1030,7 +1003,7 @@ static size_t prop_scratch_bytes(bpmh_plan plan, const PropCall& c) {
   if (c.mean_mode != BPMH_MEAN_WEIGHTED && !c.out_scat) return 0;
   const size_t n_w = (size_t)c.batch * plan->nx;
   const size_t n_s = c.out_scat ? (size_t)c.batch * c.nz * plan->nx : 0;
-  return (n_w + n_s) * sizeof(float2) + 2 * (((size_t)c.batch * 4 + 255) & ~(size_t)255);
+  return (n_w + n_s) * sizeof(float2) + (((size_t)c.batch * 4 + 255) & ~(size_t)255);
 }
 
 static StepScratch prop_scratch_carve(bpmh_plan plan, const PropCall& c, void* base) {
@@ -1039,9 +1012,8 @@ static StepScratch prop_scratch_carve(bpmh_plan plan, const PropCall& c, void* b
   const size_t n_s = c.out_scat ? (size_t)c.batch * c.nz * plan->nx : 0;
   const size_t seg = ((size_t)c.batch * 4 + 255) & ~(size_t)255;
   char* p = (char*)base;
-  r.cnt = (unsigned*)p;
-  r.dn0 = (float*)(p + seg);
-  r.wpart = (float2*)(p + 2 * seg);
+  r.dn0 = (float*)p;
+  r.wpart = (float2*)(p + seg);
   r.spart = r.wpart + n_w;
   (void)n_s;
   return r;
@@ -1096,7 +1068,6 @@ static int prop_run(bpmh_plan plan, const PropCall& c, const StepScratch& sc, hi
   ca.dn_kind = c.dn_kind;
   ca.wpart = sc.wpart;
   ca.dn0_item = sc.dn0;
-  ca.wcount = sc.cnt;
   ca.sw = c.out_scat ? c.scat_w : nullptr;
   ca.gw = c.g_w;
   ca.pw = c.plain;
@@ -1255,11 +1226,11 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane, c
   rc = plan_upload(plan);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
-  StepScratch sc = {nullptr, nullptr, nullptr, nullptr};
+  StepScratch sc = {nullptr, nullptr, nullptr};
   const size_t sb = prop_scratch_bytes(plan, c);
   if (sb) {
     void* scr = nullptr;
-    rc = plan_scratch(plan, sb, s, &scr);
+    rc = plan_scratch(plan, sb, &scr);
     if (rc) return rc;
     sc = prop_scratch_carve(plan, c, scr);
   }

@@ -24,6 +24,7 @@
 #include <cstdint>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 int bpmh_set_error(int code, const char* msg);
 
@@ -43,6 +44,9 @@ struct DwArgs {
   float2 *ex, *ey, *ez;
   float* I;
   const float *px, *py, *pz;  // points mode
+  // full-grid mode (0 = off): the grid above is the octant of an (mz0, my0,
+  // mx0) grid centred on the focus, and the outputs are that full grid
+  int mx0, my0, mz0;
 };
 
 // coordinate along one axis: a1 + i*(a2-a1)/(n-1.f) (psf_debye.cl:46-48)
@@ -126,6 +130,56 @@ BPMH_DEV void combine(const DwArgs& a, float phi, float2 I0, float2 I1, float2 I
     const float vy = ey.x * ey.x + ey.y * ey.y;
     const float vz = ez.x * ez.x + ez.y * ez.y;
     a.I[o] = a.i_mode ? vx : vx + vy + vz;
+  }
+}
+
+// Full-grid form of combine: the value of octant point (i, j, z) goes to every
+// full-grid index f = N0/2 -+ o along each axis that lies in the grid (the
+// index map of _focus_fields_debye.py:116-143), the fields conjugated for
+// f_z <= Nz0/2; streaming 8-byte stores (the outputs are written once).
+BPMH_DEV void combine_full(const DwArgs& a, float phi, float2 I0, float2 I1, float2 I2, int i,
+                           int j, int z) {
+  double s2d, c2d, s1d, c1d;
+  const float phi2 = f_mul(2.f, phi);
+  sincos((double)phi2, &s2d, &c2d);
+  sincos((double)phi, &s1d, &c1d);
+  const float c2 = (float)c2d, s2 = (float)s2d, cp = (float)c1d, sp = (float)s1d;
+  const float2 ex = make_float2(a.ex0 * (I0.x + I2.x * c2) + (a.ey0 * s2) * I2.x,
+                                a.ex0 * (I0.y + I2.y * c2) + (a.ey0 * s2) * I2.y);
+  const float2 ey = make_float2(a.ey0 * (I0.x - I2.x * c2) + (a.ex0 * s2) * I2.x,
+                                a.ey0 * (I0.y - I2.y * c2) + (a.ex0 * s2) * I2.y);
+  const float m = a.ex0 * cp + a.ey0 * sp;
+  const float2 ez = make_float2((2.f * I1.y) * m, (-2.f * I1.x) * m);
+  const float vx = ex.x * ex.x + ex.y * ex.y;
+  const float vy = ey.x * ey.x + ey.y * ey.y;
+  const float vz = ez.x * ez.x + ez.y * ez.y;
+  const float I = a.i_mode ? vx : vx + vy + vz;
+  const int hx = a.mx0 / 2, hy = a.my0 / 2, hz = a.mz0 / 2;
+  const int fxs[2] = {hx - i, hx + i}, fys[2] = {hy - j, hy + j}, fzs[2] = {hz - z, hz + z};
+  const size_t plane = (size_t)a.my0 * a.mx0;
+#pragma unroll
+  for (int qz = 0; qz < 2; ++qz) {
+    const int fz = fzs[qz];
+    if ((qz == 1 && z == 0) || fz >= a.mz0) continue;
+    const bool conj = fz <= hz;
+    const float2 cx = conj ? make_float2(ex.x, -ex.y) : ex;
+    const float2 cy = conj ? make_float2(ey.x, -ey.y) : ey;
+    const float2 cz = conj ? make_float2(ez.x, -ez.y) : ez;
+#pragma unroll
+    for (int qy = 0; qy < 2; ++qy) {
+      const int fy = fys[qy];
+      if ((qy == 1 && j == 0) || fy >= a.my0) continue;
+#pragma unroll
+      for (int qx = 0; qx < 2; ++qx) {
+        const int fx = fxs[qx];
+        if ((qx == 1 && i == 0) || fx >= a.mx0) continue;
+        const size_t o = (size_t)fz * plane + (size_t)fy * a.mx0 + fx;
+        if (a.ex) st_stream(a.ex + o, cx);
+        if (a.ey) st_stream(a.ey + o, cy);
+        if (a.ez) st_stream(a.ez + o, cz);
+        if (a.I) st_stream(a.I + o, I);
+      }
+    }
   }
 }
 
@@ -329,8 +383,13 @@ BPMH_DEV void dw_mfma_tiles(const DwArgs& a, int ncols, int zp, int tp, int cp, 
         const float2 I0 = make_float2(acc[r][t][0][q], acc[r][t][1][q]);
         const float2 I1 = make_float2(acc[r][t][2][q], acc[r][t][3][q]);
         const float2 I2 = make_float2(acc[r][t][4][q], acc[r][t][5][q]);
-        combine(a, phi1, I0, I1, I2, (size_t)z * out_plane + o1);
-        if (second) combine(a, phi2, I0, I1, I2, (size_t)z * out_plane + o2);
+        if (a.mz0) {
+          combine_full(a, phi1, I0, I1, I2, i, j, z);
+          if (second) combine_full(a, phi2, I0, I1, I2, j, i, z);
+        } else {
+          combine(a, phi1, I0, I1, I2, (size_t)z * out_plane + o1);
+          if (second) combine(a, phi2, I0, I1, I2, (size_t)z * out_plane + o2);
+        }
       }
   }
 }
@@ -498,20 +557,65 @@ static hipError_t table_pool(hipMemPool_t* out) {
   return hipSuccess;
 }
 
-extern "C" int bpmh_debye_wolf(int nx, int ny, int nz, float x1, float x2, float y1, float y2,
-                               float z1, float z2, float lam, const float* alphas, int n_alphas,
-                               int int_steps, float ex0, float ey0, int variant, float sig,
-                               int i_mode, float* ex, float* ey, float* ez, float* I,
-                               void* stream) {
-  if (nx < 1 || ny < 1 || nz < 1) return bpmh_set_error(BPMH_ERR_INVALID, "bad grid");
-  if ((long long)nx * ny > (1LL << 31) - 1) return bpmh_set_error(BPMH_ERR_INVALID, "grid too large");
-  DwArgs a;
-  int rc = fill_args(a, lam, alphas, n_alphas, int_steps, ex0, ey0, variant, sig, i_mode);
-  if (rc) return rc;
-  a.nx = nx; a.ny = ny; a.nz = nz;
-  a.x1 = x1; a.x2 = x2; a.y1 = y1; a.y2 = y2; a.z1 = z1; a.z2 = z2;
-  a.ex = (float2*)ex; a.ey = (float2*)ey; a.ez = (float2*)ez; a.I = I;
-  hipStream_t st = (hipStream_t)stream;
+// Optional HIP-event timing of the grid path's kernels (this thread's calls):
+// kinds 0 parameters, 1 sample table, 2 phase table, 3 product (with the
+// epilogue that writes the outputs).
+struct DwTiming {
+  bool on = false;
+  std::vector<hipEvent_t> ev;  // pairs
+  std::vector<int> kind;
+  size_t used = 0;
+};
+static thread_local DwTiming g_dwt;
+
+extern "C" int bpmh_debye_set_timing(int enable) {
+  g_dwt.on = enable != 0;
+  g_dwt.used = 0;
+  return BPMH_OK;
+}
+
+extern "C" int bpmh_debye_kernel_times(float* ms, int* counts) {
+  if (!ms || !counts) return bpmh_set_error(BPMH_ERR_INVALID, "null argument");
+  for (int k = 0; k < 4; ++k) {
+    ms[k] = 0.f;
+    counts[k] = 0;
+  }
+  for (size_t i = 0; i < g_dwt.used; ++i) {
+    float t = 0.f;
+    hipError_t e = hipEventSynchronize(g_dwt.ev[2 * i + 1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, g_dwt.ev[2 * i], g_dwt.ev[2 * i + 1]);
+    if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(e));
+    ms[g_dwt.kind[i]] += t;
+    counts[g_dwt.kind[i]] += 1;
+  }
+  g_dwt.used = 0;
+  return BPMH_OK;
+}
+
+// Run `launch` on st, between an event pair when timing is on.
+template <class F>
+static int dw_timed(int kind, hipStream_t st, F&& launch) {
+  if (!g_dwt.on) {
+    launch();
+    return BPMH_OK;
+  }
+  while (g_dwt.ev.size() < 2 * (g_dwt.used + 1)) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return bpmh_set_error(BPMH_ERR_HIP, "hipEventCreate failed");
+    g_dwt.ev.push_back(e);
+  }
+  if (g_dwt.kind.size() < g_dwt.used + 1) g_dwt.kind.resize(g_dwt.used + 1);
+  const size_t i = g_dwt.used++;
+  g_dwt.kind[i] = kind;
+  (void)hipEventRecord(g_dwt.ev[2 * i], st);
+  launch();
+  (void)hipEventRecord(g_dwt.ev[2 * i + 1], st);
+  return BPMH_OK;
+}
+
+// Tables and product for the grid in `a` (octant or whole grid) on stream st.
+static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st) {
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int ncols = nx * ny;
   // samples index the y extent of two grids (at most 65535); the product
   // grid's y extent is about nz / 128
@@ -523,7 +627,7 @@ extern "C" int bpmh_debye_wolf(int nx, int ny, int nz, float x1, float x2, float
   // stream-ordered scratch, zero-padded to whole product tiles: A (3 x tp x
   // cp), co (tp), P (tp x zp), per-sample parameters (3 x total)
   // identical x and y axes: the table and the product run over column pairs
-  const bool pairs = nx == ny && x1 == y1 && x2 == y2;
+  const bool pairs = nx == ny && a.x1 == a.y1 && a.x2 == a.y2;
   const int ntab = pairs ? nx * (nx + 1) / 2 : ncols;  // table columns
   const int tp = (total + 15) / 16 * 16;
   const int nzt = (nz + 15) / 16, zp = nzt * 16;
@@ -541,25 +645,76 @@ extern "C" int bpmh_debye_wolf(int nx, int ny, int nz, float x1, float x2, float
   float2* P = (float2*)(((uintptr_t)(A + nA) + 15) & ~(uintptr_t)15);
   float* co = (float*)(P + nP);
   float* par = co + tp;
-  hipLaunchKernelGGL(k_dw_params, dim3((total + 63) / 64), dim3(64), 0, st, a, total, par);
-  if (pairs)
-    hipLaunchKernelGGL(k_dw_samples_pairs, dim3((cp + 255) / 256, tp), dim3(256), 0, st, a, total, ntab, tp,
-                       cp, (const float*)par, co, A);
-  else
-    hipLaunchKernelGGL(k_dw_samples, dim3((cp + 255) / 256, tp), dim3(256), 0, st, a, total, ncols, tp, cp,
-                       (const float*)par, co, A);
-  hipLaunchKernelGGL(k_dw_phases, dim3((zp + 63) / 64, tp), dim3(64), 0, st, a, total, tp, zp, (const float*)co, P);
+  int rc = dw_timed(0, st, [&] {
+    hipLaunchKernelGGL(k_dw_params, dim3((total + 63) / 64), dim3(64), 0, st, a, total, par);
+  });
+  if (!rc) rc = dw_timed(1, st, [&] {
+    if (pairs)
+      hipLaunchKernelGGL(k_dw_samples_pairs, dim3((cp + 255) / 256, tp), dim3(256), 0, st, a, total,
+                         ntab, tp, cp, (const float*)par, co, A);
+    else
+      hipLaunchKernelGGL(k_dw_samples, dim3((cp + 255) / 256, tp), dim3(256), 0, st, a, total, ncols,
+                         tp, cp, (const float*)par, co, A);
+  });
+  if (!rc) rc = dw_timed(2, st, [&] {
+    hipLaunchKernelGGL(k_dw_phases, dim3((zp + 63) / 64, tp), dim3(64), 0, st, a, total, tp, zp,
+                       (const float*)co, P);
+  });
   const dim3 mgrid(cp / (16 * DM_CT), (nzt + 3) / 4);
-  if (pairs)
-    hipLaunchKernelGGL(k_dw_mfma<true>, mgrid, dim3(256), 0, st, a, ntab, nzt, tp, cp, (const float2*)P,
-                       (const float*)A);
-  else
-    hipLaunchKernelGGL(k_dw_mfma<false>, mgrid, dim3(256), 0, st, a, ncols, nzt, tp, cp, (const float2*)P,
-                       (const float*)A);
+  if (!rc) rc = dw_timed(3, st, [&] {
+    if (pairs)
+      hipLaunchKernelGGL(k_dw_mfma<true>, mgrid, dim3(256), 0, st, a, ntab, nzt, tp, cp,
+                         (const float2*)P, (const float*)A);
+    else
+      hipLaunchKernelGGL(k_dw_mfma<false>, mgrid, dim3(256), 0, st, a, ncols, nzt, tp, cp,
+                         (const float2*)P, (const float*)A);
+  });
+  if (rc) {
+    (void)hipFreeAsync(scr, st);
+    return rc;
+  }
   e = hipGetLastError();
   (void)hipFreeAsync(scr, st);
   if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(e));
   return BPMH_OK;
+}
+
+extern "C" int bpmh_debye_wolf(int nx, int ny, int nz, float x1, float x2, float y1, float y2,
+                               float z1, float z2, float lam, const float* alphas, int n_alphas,
+                               int int_steps, float ex0, float ey0, int variant, float sig,
+                               int i_mode, float* ex, float* ey, float* ez, float* I,
+                               void* stream) {
+  if (nx < 1 || ny < 1 || nz < 1) return bpmh_set_error(BPMH_ERR_INVALID, "bad grid");
+  if ((long long)nx * ny > (1LL << 31) - 1) return bpmh_set_error(BPMH_ERR_INVALID, "grid too large");
+  DwArgs a;
+  int rc = fill_args(a, lam, alphas, n_alphas, int_steps, ex0, ey0, variant, sig, i_mode);
+  if (rc) return rc;
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  a.x1 = x1; a.x2 = x2; a.y1 = y1; a.y2 = y2; a.z1 = z1; a.z2 = z2;
+  a.ex = (float2*)ex; a.ey = (float2*)ey; a.ez = (float2*)ez; a.I = I;
+  return debye_grid_run(a, int_steps, n_alphas, (hipStream_t)stream);
+}
+
+extern "C" int bpmh_focus_field_debye(int nx0, int ny0, int nz0, double dx, double dy, double dz,
+                                      float lam, const float* alphas, int n_alphas,
+                                      int int_steps, float* ex, float* ey, float* ez, float* I,
+                                      void* stream) {
+  if (nx0 < 2 || ny0 < 2 || nz0 < 2 || (nx0 | ny0 | nz0) & 1)
+    return bpmh_set_error(BPMH_ERR_UNSUPPORTED, "odd shapes not supported yet");
+  if ((long long)nx0 * ny0 * nz0 > (1LL << 40)) return bpmh_set_error(BPMH_ERR_INVALID, "grid too large");
+  DwArgs a;
+  int rc = fill_args(a, lam, alphas, n_alphas, int_steps, 1.f, 0.f, BPMH_DW_INDEX, 0.f, 0);
+  if (rc) return rc;
+  // the octant (nz0/2+1, ny0/2+1, nx0/2+1) at x = 0 .. (nx0/2)*dx, the
+  // extents of _focus_fields_debye.py:102-109 formed in float64 and passed
+  // as float32 kernel arguments
+  a.nx = nx0 / 2 + 1; a.ny = ny0 / 2 + 1; a.nz = nz0 / 2 + 1;
+  a.x1 = 0.f; a.x2 = (float)(dx * (a.nx - 1.));
+  a.y1 = 0.f; a.y2 = (float)(dy * (a.ny - 1.));
+  a.z1 = 0.f; a.z2 = (float)(dz * (a.nz - 1.));
+  a.mx0 = nx0; a.my0 = ny0; a.mz0 = nz0;
+  a.ex = (float2*)ex; a.ey = (float2*)ey; a.ez = (float2*)ez; a.I = I;
+  return debye_grid_run(a, int_steps, n_alphas, (hipStream_t)stream);
 }
 
 extern "C" int bpmh_debye_wolf_points(int n, const float* x, const float* y, const float* z,

@@ -52,17 +52,15 @@ def focus_field_debye_device(shape, units, lam, NA, n0=1., n_integration_steps=2
     if not Nx0 % 2 + Ny0 % 2 + Nz0 % 2 == 0:
         raise NotImplementedError("odd shapes not supported yet")
     alphas = _alphas(NA, n0)
-    Nx, Ny, Nz = Nx0 // 2 + 1, Ny0 // 2 + 1, Nz0 // 2 + 1
-    uo, exo, eyo, ezo = _grid_call(Nx, Ny, Nz, 0., dx * (Nx - 1.), 0., dy * (Ny - 1.), 0.,
-                                   dz * (Nz - 1.), 1. * lam / n0, alphas, n_integration_steps,
-                                   _lib.DW_INDEX)
     u = _lib.empty((Nz0, Ny0, Nx0), np.float32)
     ex = _lib.empty((Nz0, Ny0, Nx0), np.complex64)
     ey = _lib.empty((Nz0, Ny0, Nx0), np.complex64)
     ez = _lib.empty((Nz0, Ny0, Nx0), np.complex64)
-    _lib.call("bpmh_debye_mirror", int(Nx0), int(Ny0), int(Nz0), _lib.ptr(uo), _lib.ptr(exo),
-              _lib.ptr(eyo), _lib.ptr(ezo), _lib.ptr(u), _lib.ptr(ex), _lib.ptr(ey), _lib.ptr(ez),
-              _lib.stream_handle())
+    a, ap = _alpha_arg(alphas)
+    # the octant's quadrature with each value written to its mirror positions
+    _lib.call("bpmh_focus_field_debye", int(Nx0), int(Ny0), int(Nz0), float(dx), float(dy),
+              float(dz), float(np.float32(1. * lam / n0)), ap, len(a), int(n_integration_steps),
+              _lib.ptr(ex), _lib.ptr(ey), _lib.ptr(ez), _lib.ptr(u), _lib.stream_handle())
     return u, ex, ey, ez
 
 

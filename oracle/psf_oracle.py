@@ -31,8 +31,28 @@ mirror_octant     focus_field_debye host mirror  _focus_fields_debye.py:116-143
                   the TIFF profile fixtures of the reference
                   (tests/data/psf_NA_0.8_dx_0.1_n_*.tif, central lines).
 focus_field_debye _focus_fields_debye.py:48-146 -- composition of the above.
+focus_field_debye_gauss  debye_wolf_gauss psf_debye.cl:273-362, host
+                  _focus_fields_debye.py:361-441
+                  pinned: after removing the quirk samples one by one, equal
+                  to scipy quad of the sin-weighted integrand times the pupil
+                  exp(-t^2/(2 amax^2 sig^2)) per annulus, with the residual
+                  falling as dt^2 (the trapezoid rule's own error); the host
+                  entry's lam, arcsin(NAs) and octant coordinates.
+_mirror_half_offset  focus_field_debye3 / _gauss mirrors
+                  _focus_fields_debye.py:282-304, :425-439
+                  pinned: element-by-element index map from the per-axis rule
+                  (low half reversed, high half in order, overlap written by
+                  the reversed copy; conjugation on the high z half for
+                  debye3 only) on even, odd and mixed grids, and for even
+                  grids the physical |x| map.
+focus_field_debye3  _focus_fields_debye.py:218-310 -- composition of the
+                  index quadrature (pinned above) and _mirror_half_offset.
 cylindrical       psf_cylindrical[_plane] kernels psf_cylindrical.cl:74-185
-                  pinned: trapezoid on [-alpha, alpha+dt] vs scipy quad.
+                  pinned: trapezoid on [-alpha, alpha+dt] vs scipy quad; the
+                  plane variant's float-equality end weights against the
+                  index variant plus single-sample corrections (both the
+                  case where t_{S-1} lands on alpha and the case where it
+                  misses).
 """
 import numpy as np
 from scipy.special import jv

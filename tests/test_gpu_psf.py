@@ -203,3 +203,33 @@ def test_plane_variant_fills_a_provided_tensor(bpm):
     assert np.array_equal(g.cpu().numpy(), ref)
     with pytest.raises(ValueError):
         Q.focus_field_debye_plane((32, 24), (.1, .1), ex_g=torch.zeros((24, 32), device="cuda"))
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 64), (48, 32, 40), (256, 256, 256)])
+def test_fused_focus_field_equals_octant_plus_mirror(bpm, shape):
+    """bpmh_focus_field_debye (the product's epilogue writes every octant
+    value to its mirror positions) against bpmh_debye_wolf on the octant
+    followed by bpmh_debye_mirror: the same values bit for bit, for the
+    pair-table path (square x/y axes) and the general one, and at the 256^3
+    workload size."""
+    import torch
+    from bpm_amd import focus_fields as Q
+    from bpm_amd import _lib
+    say("case fused Debye", shape)
+    Nx0, Ny0, Nz0 = shape
+    units, lam, NA, n0, S = (.05, .06, .05), .5, 1.1, 1.33, 128
+    if shape == (64, 64, 64) or shape == (256, 256, 256):
+        units = (.05,) * 3
+    dx, dy, dz = units
+    alphas = Q._alphas(NA, n0)
+    Nx, Ny, Nz = Nx0 // 2 + 1, Ny0 // 2 + 1, Nz0 // 2 + 1
+    oct_ = Q._grid_call(Nx, Ny, Nz, 0., dx * (Nx - 1.), 0., dy * (Ny - 1.), 0., dz * (Nz - 1.),
+                        1. * lam / n0, alphas, S, _lib.DW_INDEX)
+    ref = [_lib.empty((Nz0, Ny0, Nx0), np.float32)] + \
+          [_lib.empty((Nz0, Ny0, Nx0), np.complex64) for _ in range(3)]
+    _lib.call("bpmh_debye_mirror", Nx0, Ny0, Nz0, *[_lib.ptr(o) for o in oct_],
+              *[_lib.ptr(r) for r in ref], _lib.stream_handle())
+    got = Q.focus_field_debye_device(shape, units, lam, NA, n0, S)
+    torch.cuda.synchronize()
+    for g, r in zip(got, ref):
+        assert torch.equal(g, r)
