@@ -130,8 +130,8 @@ def debye_extra(torch):
     lib = _lib.load()
     _lib.check(lib.bpmh_debye_set_timing(1))
     Q.focus_field_debye_device(shape, units, lam, NA, n0, S)
-    ms = (ctypes.c_float * 4)()
-    cnt = (ctypes.c_int * 4)()
+    ms = (ctypes.c_float * 5)()
+    cnt = (ctypes.c_int * 5)()
     _lib.check(lib.bpmh_debye_kernel_times(ms, cnt))
     _lib.check(lib.bpmh_debye_set_timing(0))
     # the product: for every (plane, table column, sample) 6 real
@@ -147,13 +147,14 @@ def debye_extra(torch):
     return {"workload": "Debye-Wolf 256^3, NA 1.1, n0 1.33, 512 theta steps, x-polarised",
             "ms": dt * 1e3, "theta_samples_per_s": samples / dt,
             "kernel_us": {"params": ms[0] * 1e3, "sample_table": ms[1] * 1e3,
-                          "phase_table": ms[2] * 1e3, "product_and_outputs": ms[3] * 1e3},
+                          "phase_table": ms[2] * 1e3, "product": ms[3] * 1e3,
+                          "full_grid_spread": ms[4] * 1e3},
             "product_mfma_tflops": mfma_tf, "product_mfma_frac": mfma_tf / FP32_PEAK_TFLOPS,
             "product_useful_tflops": useful_tf,
             "note": "MFMA fraction = flops the MFMA units execute over the padded "
                     "144 x 8416 x 528 product (12 per plane, column pair and sample) / the "
-                    "product kernel's time (which includes writing the 470 MB of outputs) / "
-                    "157.3 TFLOP/s dense FP32 matrix peak; useful = unpadded 129 x 8385 x 513"}
+                    "product kernel's time (event pair) / 157.3 TFLOP/s dense FP32 matrix "
+                    "peak; useful = unpadded 129 x 8385 x 513"}
 
 
 def served_extra(seed, n_jobs=200, load=0.8, plain_jobs=48, runs=2):

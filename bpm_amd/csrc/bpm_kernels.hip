@@ -22,6 +22,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -588,8 +589,8 @@ struct bpmh_plan_s {
   std::vector<float2> htw_x, htw_y;  // stage-major twiddles (host)
   float2* tw_x;                      // their device copies (first propagation)
   float2* tw_y;
-  void* scratch;
-  size_t scratch_bytes;
+  void* work;                        // step scratch (allocated on demand)
+  size_t work_bytes;
   int device;
   // timing: events recorded around every launch of bpmh_propagate
   int timing;
@@ -697,6 +698,8 @@ static void make_stage_twiddles(int N, int E, std::vector<float2>& out) {
   if (out.empty()) out.push_back(make_float2(1.f, 0.f));
 }
 
+static int plan_upload(bpmh_plan p);
+
 extern "C" int bpmh_plan_create(int nx, int ny, bpmh_plan* out) {
   if (!out) return bpmh_set_error(BPMH_ERR_INVALID, "null plan pointer");
   *out = nullptr;
@@ -706,8 +709,8 @@ extern "C" int bpmh_plan_create(int nx, int ny, bpmh_plan* out) {
   bpmh_plan p = new bpmh_plan_s();
   p->nx = nx;
   p->ny = ny;
-  p->scratch = nullptr;
-  p->scratch_bytes = 0;
+  p->work = nullptr;
+  p->work_bytes = 0;
   p->timing = 0;
   p->ev_used = 0;
   p->row_v = 1;
@@ -720,13 +723,25 @@ extern "C" int bpmh_plan_create(int nx, int ny, bpmh_plan* out) {
   p->tw_y = nullptr;
   make_stage_twiddles(nx, row_e(nx), p->htw_x);
   make_stage_twiddles(ny, col_e(ny), p->htw_y);
+  // device copies now when a device is present (on a machine without one
+  // the plan stays host-only: its argument checks still run)
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0) {
+    const int rc = plan_upload(p);
+    if (rc) {
+      delete p;
+      return rc;
+    }
+  } else {
+    (void)hipGetLastError();
+  }
   *out = p;
   return BPMH_OK;
 }
 
-// Device copies of the twiddle tables, made on the plan's first propagation
-// (plan creation itself touches no device); the device current then is the
-// plan's device.
+// Device copies of the twiddle tables on the current device, which becomes
+// the plan's device (made at plan creation, or at the first propagation of a
+// plan created without a device).
 static int plan_upload(bpmh_plan p) {
   if (p->tw_x) return BPMH_OK;
   HIP_TRY(hipGetDevice(&p->device));
@@ -753,7 +768,7 @@ extern "C" int bpmh_plan_destroy(bpmh_plan p) {
   if (!p) return BPMH_OK;
   (void)hipFree(p->tw_x);
   (void)hipFree(p->tw_y);
-  if (p->scratch) (void)hipFree(p->scratch);
+  if (p->work) (void)hipFree(p->work);
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
   (void)graph_release_all(p);
   if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
@@ -806,19 +821,19 @@ static int timing_reserve(bpmh_plan p, size_t n) {
 
 // Plan scratch of at least `bytes` (grown, never shrunk).
 static int plan_scratch(bpmh_plan p, size_t bytes, void** ptr) {
-  if (bytes > p->scratch_bytes) {
-    if (p->scratch) {
+  if (bytes > p->work_bytes) {
+    if (p->work) {
       HIP_TRY(hipDeviceSynchronize());
-      HIP_TRY(hipFree(p->scratch));
-      p->scratch = nullptr;
-      p->scratch_bytes = 0;
+      HIP_TRY(hipFree(p->work));
+      p->work = nullptr;
+      p->work_bytes = 0;
     }
     bytes = (bytes + 255) & ~(size_t)255;
-    if (hipMalloc(&p->scratch, bytes) != hipSuccess)
+    if (hipMalloc(&p->work, bytes) != hipSuccess)
       return bpmh_set_error(BPMH_ERR_NOMEM, "hipMalloc failed for plan scratch");
-    p->scratch_bytes = bytes;
+    p->work_bytes = bytes;
   }
-  *ptr = p->scratch;
+  *ptr = p->work;
   return BPMH_OK;
 }
 
@@ -1020,8 +1035,10 @@ static StepScratch prop_scratch_carve(bpmh_plan plan, const PropCall& c, void* b
 }
 
 // Enqueue (or capture and replay) the whole z-loop of one call on stream s.
+// may_allocate false: plain launches only (no graph capture or timing
+// events, which create runtime objects).
 static int prop_run(bpmh_plan plan, const PropCall& c, const StepScratch& sc, hipStream_t s,
-                    bool graph_ok) {
+                    bool may_allocate) {
   const int nx = plan->nx, ny = plan->ny;
   const int batch = c.batch, nz = c.nz;
   const row_fn row = pick_row(nx, plan->row_v);
@@ -1073,11 +1090,12 @@ static int prop_run(bpmh_plan plan, const PropCall& c, const StepScratch& sc, hi
   ca.pw = c.plain;
   ca.spart = sc.spart;
 
-  if (plan->timing) {
+  const bool timing = may_allocate && plan->timing;
+  if (timing) {
     const int rc = timing_reserve(plan, (size_t)nz * 3 + 1);
     if (rc) return rc;
   }
-  const bool use_graph = graph_ok && plan->graph_mode && !plan->timing;
+  const bool use_graph = may_allocate && plan->graph_mode && !plan->timing;
   const hipStream_t user = s;
   std::vector<unsigned char> key;
   if (use_graph) {
@@ -1106,7 +1124,7 @@ static int prop_run(bpmh_plan plan, const PropCall& c, const StepScratch& sc, hi
     s = plan->cap_stream;
   }
   auto timed = [&](int kind, auto&& launch) {
-    if (!plan->timing) {
+    if (!timing) {
       launch();
       return;
     }

@@ -25,12 +25,40 @@ BPMH_HD float2 c_rot90(float2 a) {
   return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
 }
 
-// IEEE float32 arithmetic with FMA contraction disabled, used where a formula
-// of the reference has to be reproduced operation by operation.
-BPMH_DEV float f_mul(float a, float b) { return __fmul_rn(a, b); }
-BPMH_DEV float f_add(float a, float b) { return __fadd_rn(a, b); }
-BPMH_DEV float f_sub(float a, float b) { return __fsub_rn(a, b); }
-BPMH_DEV float f_div(float a, float b) { return __fdiv_rn(a, b); }
+// IEEE float32 arithmetic that is never fused into an FMA with a neighbouring
+// operation, used where a formula of the reference has to be reproduced
+// operation by operation (HIP's __fmul_rn / __fadd_rn are plain operators
+// that the compiler contracts; the pragma marks these operations as not
+// contractible, so each rounds on its own in every kernel that inlines
+// them).
+BPMH_DEV float f_mul(float a, float b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+BPMH_DEV float f_add(float a, float b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+BPMH_DEV float f_sub(float a, float b) {
+#pragma clang fp contract(off)
+  return a - b;
+}
+BPMH_DEV float f_div(float a, float b) {
+#pragma clang fp contract(off)
+  return a / b;
+}
+BPMH_DEV double d_mul(double a, double b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+BPMH_DEV double d_add(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+BPMH_DEV double d_sub(double a, double b) {
+#pragma clang fp contract(off)
+  return a - b;
+}
 
 // cos/sin of a float32 angle, evaluated in float64 and rounded to float32:
 // bit-reproducible against a float64 libm for all but ~2^-28 of inputs.

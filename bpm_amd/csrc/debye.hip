@@ -44,9 +44,6 @@ struct DwArgs {
   float2 *ex, *ey, *ez;
   float* I;
   const float *px, *py, *pz;  // points mode
-  // full-grid mode (0 = off): the grid above is the octant of an (mz0, my0,
-  // mx0) grid centred on the focus, and the outputs are that full grid
-  int mx0, my0, mz0;
 };
 
 // coordinate along one axis: a1 + i*(a2-a1)/(n-1.f) (psf_debye.cl:46-48)
@@ -109,78 +106,41 @@ BPMH_DEV Sample make_sample(const DwArgs& a, int i_n, int i_t, float kr) {
   return sample_amps(sample_params(a, i_n, i_t), kr);
 }
 
-// Ex, Ey, Ez, I from the three integrals (psf_debye.cl:88-109)
-BPMH_DEV void combine(const DwArgs& a, float phi, float2 I0, float2 I1, float2 I2, size_t o) {
+// Ex, Ey, Ez, I from the three integrals (psf_debye.cl:88-109), every
+// product and sum rounded on its own (no contraction), so each kernel that
+// inlines this gives the same bits.
+struct DwOut {
+  float2 ex, ey, ez;
+  float I;
+};
+
+BPMH_DEV DwOut dw_fields(const DwArgs& a, float phi, float2 I0, float2 I1, float2 I2) {
   double s2d, c2d, s1d, c1d;
   const float phi2 = f_mul(2.f, phi);
   sincos((double)phi2, &s2d, &c2d);
   sincos((double)phi, &s1d, &c1d);
   const float c2 = (float)c2d, s2 = (float)s2d, cp = (float)c1d, sp = (float)s1d;
-  const float2 ex = make_float2(a.ex0 * (I0.x + I2.x * c2) + (a.ey0 * s2) * I2.x,
-                                a.ex0 * (I0.y + I2.y * c2) + (a.ey0 * s2) * I2.y);
-  const float2 ey = make_float2(a.ey0 * (I0.x - I2.x * c2) + (a.ex0 * s2) * I2.x,
-                                a.ey0 * (I0.y - I2.y * c2) + (a.ex0 * s2) * I2.y);
-  const float m = a.ex0 * cp + a.ey0 * sp;
-  const float2 ez = make_float2((2.f * I1.y) * m, (-2.f * I1.x) * m);
-  if (a.ex) a.ex[o] = ex;
-  if (a.ey) a.ey[o] = ey;
-  if (a.ez) a.ez[o] = ez;
-  if (a.I) {
-    const float vx = ex.x * ex.x + ex.y * ex.y;
-    const float vy = ey.x * ey.x + ey.y * ey.y;
-    const float vz = ez.x * ez.x + ez.y * ez.y;
-    a.I[o] = a.i_mode ? vx : vx + vy + vz;
-  }
+  const float ey0s2 = f_mul(a.ey0, s2), ex0s2 = f_mul(a.ex0, s2);
+  DwOut o;
+  o.ex = make_float2(f_add(f_mul(a.ex0, f_add(I0.x, f_mul(I2.x, c2))), f_mul(ey0s2, I2.x)),
+                     f_add(f_mul(a.ex0, f_add(I0.y, f_mul(I2.y, c2))), f_mul(ey0s2, I2.y)));
+  o.ey = make_float2(f_add(f_mul(a.ey0, f_sub(I0.x, f_mul(I2.x, c2))), f_mul(ex0s2, I2.x)),
+                     f_add(f_mul(a.ey0, f_sub(I0.y, f_mul(I2.y, c2))), f_mul(ex0s2, I2.y)));
+  const float m = f_add(f_mul(a.ex0, cp), f_mul(a.ey0, sp));
+  o.ez = make_float2(f_mul(f_mul(2.f, I1.y), m), f_mul(f_mul(-2.f, I1.x), m));
+  const float vx = f_add(f_mul(o.ex.x, o.ex.x), f_mul(o.ex.y, o.ex.y));
+  const float vy = f_add(f_mul(o.ey.x, o.ey.x), f_mul(o.ey.y, o.ey.y));
+  const float vz = f_add(f_mul(o.ez.x, o.ez.x), f_mul(o.ez.y, o.ez.y));
+  o.I = a.i_mode ? vx : f_add(f_add(vx, vy), vz);
+  return o;
 }
 
-// Full-grid form of combine: the value of octant point (i, j, z) goes to every
-// full-grid index f = N0/2 -+ o along each axis that lies in the grid (the
-// index map of _focus_fields_debye.py:116-143), the fields conjugated for
-// f_z <= Nz0/2; streaming 8-byte stores (the outputs are written once).
-BPMH_DEV void combine_full(const DwArgs& a, float phi, float2 I0, float2 I1, float2 I2, int i,
-                           int j, int z) {
-  double s2d, c2d, s1d, c1d;
-  const float phi2 = f_mul(2.f, phi);
-  sincos((double)phi2, &s2d, &c2d);
-  sincos((double)phi, &s1d, &c1d);
-  const float c2 = (float)c2d, s2 = (float)s2d, cp = (float)c1d, sp = (float)s1d;
-  const float2 ex = make_float2(a.ex0 * (I0.x + I2.x * c2) + (a.ey0 * s2) * I2.x,
-                                a.ex0 * (I0.y + I2.y * c2) + (a.ey0 * s2) * I2.y);
-  const float2 ey = make_float2(a.ey0 * (I0.x - I2.x * c2) + (a.ex0 * s2) * I2.x,
-                                a.ey0 * (I0.y - I2.y * c2) + (a.ex0 * s2) * I2.y);
-  const float m = a.ex0 * cp + a.ey0 * sp;
-  const float2 ez = make_float2((2.f * I1.y) * m, (-2.f * I1.x) * m);
-  const float vx = ex.x * ex.x + ex.y * ex.y;
-  const float vy = ey.x * ey.x + ey.y * ey.y;
-  const float vz = ez.x * ez.x + ez.y * ez.y;
-  const float I = a.i_mode ? vx : vx + vy + vz;
-  const int hx = a.mx0 / 2, hy = a.my0 / 2, hz = a.mz0 / 2;
-  const int fxs[2] = {hx - i, hx + i}, fys[2] = {hy - j, hy + j}, fzs[2] = {hz - z, hz + z};
-  const size_t plane = (size_t)a.my0 * a.mx0;
-#pragma unroll
-  for (int qz = 0; qz < 2; ++qz) {
-    const int fz = fzs[qz];
-    if ((qz == 1 && z == 0) || fz >= a.mz0) continue;
-    const bool conj = fz <= hz;
-    const float2 cx = conj ? make_float2(ex.x, -ex.y) : ex;
-    const float2 cy = conj ? make_float2(ey.x, -ey.y) : ey;
-    const float2 cz = conj ? make_float2(ez.x, -ez.y) : ez;
-#pragma unroll
-    for (int qy = 0; qy < 2; ++qy) {
-      const int fy = fys[qy];
-      if ((qy == 1 && j == 0) || fy >= a.my0) continue;
-#pragma unroll
-      for (int qx = 0; qx < 2; ++qx) {
-        const int fx = fxs[qx];
-        if ((qx == 1 && i == 0) || fx >= a.mx0) continue;
-        const size_t o = (size_t)fz * plane + (size_t)fy * a.mx0 + fx;
-        if (a.ex) st_stream(a.ex + o, cx);
-        if (a.ey) st_stream(a.ey + o, cy);
-        if (a.ez) st_stream(a.ez + o, cz);
-        if (a.I) st_stream(a.I + o, I);
-      }
-    }
-  }
+BPMH_DEV void combine(const DwArgs& a, float phi, float2 I0, float2 I1, float2 I2, size_t o) {
+  const DwOut v = dw_fields(a, phi, I0, I1, I2);
+  if (a.ex) a.ex[o] = v.ex;
+  if (a.ey) a.ey[o] = v.ey;
+  if (a.ez) a.ez[o] = v.ez;
+  if (a.I) a.I[o] = v.I;
 }
 
 // ----------------------------------------------------------------------------
@@ -383,13 +343,8 @@ BPMH_DEV void dw_mfma_tiles(const DwArgs& a, int ncols, int zp, int tp, int cp, 
         const float2 I0 = make_float2(acc[r][t][0][q], acc[r][t][1][q]);
         const float2 I1 = make_float2(acc[r][t][2][q], acc[r][t][3][q]);
         const float2 I2 = make_float2(acc[r][t][4][q], acc[r][t][5][q]);
-        if (a.mz0) {
-          combine_full(a, phi1, I0, I1, I2, i, j, z);
-          if (second) combine_full(a, phi2, I0, I1, I2, j, i, z);
-        } else {
-          combine(a, phi1, I0, I1, I2, (size_t)z * out_plane + o1);
-          if (second) combine(a, phi2, I0, I1, I2, (size_t)z * out_plane + o2);
-        }
+        combine(a, phi1, I0, I1, I2, (size_t)z * out_plane + o1);
+        if (second) combine(a, phi2, I0, I1, I2, (size_t)z * out_plane + o2);
       }
   }
 }
@@ -399,12 +354,13 @@ BPMH_DEV void dw_mfma_tiles(const DwArgs& a, int ncols, int zp, int tp, int cp, 
 // loaded column operand twice but need 220 registers, two wavefronts per
 // SIMD, and leave a tail of unequal wavefronts: 0.364 vs 0.341 ms per 256^3
 // call.)
+// The launch covers the work-group rows from zrow0 on (grid y rows).
 template <bool PAIRS>
 __global__ void __launch_bounds__(256, 2) k_dw_mfma(DwArgs a, int ncols, int nzt, int tp, int cp,
                                                 const float2* __restrict__ P,
-                                                const float* __restrict__ A) {
+                                                const float* __restrict__ A, int zrow0) {
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int zt0 = blockIdx.y * 4 + w;
+  const int zt0 = (blockIdx.y + zrow0) * 4 + w;
   if (zt0 >= nzt) return;  // whole wavefront; the kernel has no barriers
   const int c0 = blockIdx.x * (16 * DM_CT);
   dw_mfma_tiles<1, PAIRS>(a, ncols, nzt * 16, tp, cp, zt0, c0, l, P, A);
@@ -431,13 +387,25 @@ __global__ void __launch_bounds__(64) k_debye_points(DwArgs a, int n) {
   combine(a, phi, I0, I1, I2, (size_t)p);
 }
 
+// Full-grid planes of one launch of the spread: blockIdx.z < n_hi gives
+// fz = hi0 + blockIdx.z, the rest fz = lo0 - (blockIdx.z - n_hi) (the two
+// images of a range of octant planes; hi0 = 0, n_hi = nz0 covers every
+// plane).
+struct PlaneMap {
+  int hi0, n_hi, lo0;
+};
+BPMH_DEV int plane_of(PlaneMap m) {
+  const int b = blockIdx.z;
+  return b < m.n_hi ? m.hi0 + b : m.lo0 - (b - m.n_hi);
+}
+
 // Full-grid spread of the octant (_focus_fields_debye.py:116-143).
 __global__ void k_debye_mirror(int nx0, int ny0, int nz0, const float* uo, const float2* exo,
                                const float2* eyo, const float2* ezo, float* u, float2* ex,
-                               float2* ey, float2* ez) {
+                               float2* ey, float2* ez, PlaneMap pm) {
   const int fx = blockIdx.x * blockDim.x + threadIdx.x;
   const int fy = blockIdx.y;
-  const int fz = blockIdx.z;
+  const int fz = plane_of(pm);
   if (fx >= nx0) return;
   const int nx = nx0 / 2 + 1, ny = ny0 / 2 + 1;
   const int ox = abs(fx - nx0 / 2), oy = abs(fy - ny0 / 2), oz = abs(fz - nz0 / 2);
@@ -459,10 +427,10 @@ typedef float dw_f2v __attribute__((ext_vector_type(2)));
 __global__ void __launch_bounds__(128) k_debye_mirror2(int nx0, int ny0, int nz0, const float* uo,
                                                        const float2* exo, const float2* eyo,
                                                        const float2* ezo, float* u, float2* ex,
-                                                       float2* ey, float2* ez) {
+                                                       float2* ey, float2* ez, PlaneMap pm) {
   const int fx = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
   const int fy = blockIdx.y;
-  const int fz = blockIdx.z;
+  const int fz = plane_of(pm);
   if (fx >= nx0) return;
   const int nx = nx0 / 2 + 1, ny = ny0 / 2 + 1;
   const int ox0 = abs(fx - nx0 / 2), ox1 = abs(fx + 1 - nx0 / 2);
@@ -558,8 +526,8 @@ static hipError_t table_pool(hipMemPool_t* out) {
 }
 
 // Optional HIP-event timing of the grid path's kernels (this thread's calls):
-// kinds 0 parameters, 1 sample table, 2 phase table, 3 product (with the
-// epilogue that writes the outputs).
+// kinds 0 parameters, 1 sample table, 2 phase table, 3 product, 4 full-grid
+// spread.
 struct DwTiming {
   bool on = false;
   std::vector<hipEvent_t> ev;  // pairs
@@ -576,7 +544,7 @@ extern "C" int bpmh_debye_set_timing(int enable) {
 
 extern "C" int bpmh_debye_kernel_times(float* ms, int* counts) {
   if (!ms || !counts) return bpmh_set_error(BPMH_ERR_INVALID, "null argument");
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < 5; ++k) {
     ms[k] = 0.f;
     counts[k] = 0;
   }
@@ -613,8 +581,45 @@ static int dw_timed(int kind, hipStream_t st, F&& launch) {
   return BPMH_OK;
 }
 
-// Tables and product for the grid in `a` (octant or whole grid) on stream st.
-static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st) {
+// Full-grid outputs for debye_grid_run: the grid in `a` is the octant of an
+// (nz0, ny0, nx0) grid; the product writes the octant into pool scratch and
+// one spread kernel then writes these outputs.
+struct DwFull {
+  int nx0, ny0, nz0;
+  float* u;
+  float2 *ex, *ey, *ez;
+};
+
+// Launch the spread of octant planes [oz0, oz1) (both their full-grid
+// images) on stream st.
+static void launch_mirror(int nx0, int ny0, int nz0, const float* uo, const float2* exo,
+                          const float2* eyo, const float2* ezo, float* u, float2* ex, float2* ey,
+                          float2* ez, int oz0, int oz1, hipStream_t st) {
+  const int hz = nz0 / 2;
+  PlaneMap pm;
+  pm.hi0 = hz + oz0;                                   // images hz + o, o < hz
+  pm.n_hi = (oz1 < hz ? oz1 : hz) - oz0;
+  if (pm.n_hi < 0) pm.n_hi = 0;
+  const int lo_begin = oz0 > 1 ? oz0 : 1;              // images hz - o, o >= 1
+  const int n_lo = oz1 - lo_begin > 0 ? oz1 - lo_begin : 0;
+  pm.lo0 = hz - lo_begin;
+  const int planes = pm.n_hi + n_lo;
+  if (planes == 0) return;
+  const bool aligned = ((uintptr_t)u % 8 == 0) && ((uintptr_t)ex % 16 == 0) &&
+                       ((uintptr_t)ey % 16 == 0) && ((uintptr_t)ez % 16 == 0);
+  if (aligned) {
+    hipLaunchKernelGGL(k_debye_mirror2, dim3((nx0 / 2 + 127) / 128, ny0, planes), dim3(128), 0, st,
+                       nx0, ny0, nz0, uo, exo, eyo, ezo, u, ex, ey, ez, pm);
+  } else {
+    hipLaunchKernelGGL(k_debye_mirror, dim3((nx0 + 127) / 128, ny0, planes), dim3(128), 0, st,
+                       nx0, ny0, nz0, uo, exo, eyo, ezo, u, ex, ey, ez, pm);
+  }
+}
+
+// Tables and product for the grid in `a` (octant or whole grid) on stream
+// st; with `full`, the octant is then spread to the full grid.
+static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st,
+                          const DwFull* full = nullptr) {
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int ncols = nx * ny;
   // samples index the y extent of two grids (at most 65535); the product
@@ -625,7 +630,8 @@ static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st
     return bpmh_set_error(BPMH_ERR_INVALID, "grid x samples too large");
   const int total = (int)total_ll;
   // stream-ordered scratch, zero-padded to whole product tiles: A (3 x tp x
-  // cp), co (tp), P (tp x zp), per-sample parameters (3 x total)
+  // cp), co (tp), P (tp x zp), per-sample parameters (3 x total), and for
+  // the full-grid outputs the octant itself (28 B per octant voxel)
   // identical x and y axes: the table and the product run over column pairs
   const bool pairs = nx == ny && a.x1 == a.y1 && a.x2 == a.y2;
   const int ntab = pairs ? nx * (nx + 1) / 2 : ncols;  // table columns
@@ -633,18 +639,27 @@ static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st
   const int nzt = (nz + 15) / 16, zp = nzt * 16;
   const int cp = (ntab + 16 * DM_CT - 1) / (16 * DM_CT) * (16 * DM_CT);
   const size_t nA = (size_t)3 * tp * cp, nP = (size_t)tp * zp;
+  const size_t noct = full ? (size_t)nx * ny * nz : 0;
+  const size_t tab_bytes = nA * sizeof(float) + nP * sizeof(float2) + tp * sizeof(float) +
+                           3 * (size_t)total * sizeof(float) + 256;
   void* scr = nullptr;
   hipMemPool_t pool = nullptr;
   hipError_t e = table_pool(&pool);
   if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(e));
-  e = hipMallocFromPoolAsync(&scr, nA * sizeof(float) + nP * sizeof(float2) + tp * sizeof(float) +
-                                       3 * (size_t)total * sizeof(float) + 256,
-                             pool, st);
+  e = hipMallocFromPoolAsync(&scr, tab_bytes + noct * 28 + 64, pool, st);
   if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_NOMEM, "stream-ordered allocation failed for the Debye tables");
   float* A = (float*)scr;
   float2* P = (float2*)(((uintptr_t)(A + nA) + 15) & ~(uintptr_t)15);
   float* co = (float*)(P + nP);
   float* par = co + tp;
+  if (full) {
+    // octant: Ex, Ey, Ez (16-byte aligned), then I
+    float2* oex = (float2*)(((uintptr_t)scr + tab_bytes + 15) & ~(uintptr_t)15);
+    a.ex = oex;
+    a.ey = oex + noct;
+    a.ez = oex + 2 * noct;
+    a.I = (float*)(oex + 3 * noct);
+  }
   int rc = dw_timed(0, st, [&] {
     hipLaunchKernelGGL(k_dw_params, dim3((total + 63) / 64), dim3(64), 0, st, a, total, par);
   });
@@ -664,11 +679,16 @@ static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st
   if (!rc) rc = dw_timed(3, st, [&] {
     if (pairs)
       hipLaunchKernelGGL(k_dw_mfma<true>, mgrid, dim3(256), 0, st, a, ntab, nzt, tp, cp,
-                         (const float2*)P, (const float*)A);
+                         (const float2*)P, (const float*)A, 0);
     else
       hipLaunchKernelGGL(k_dw_mfma<false>, mgrid, dim3(256), 0, st, a, ncols, nzt, tp, cp,
-                         (const float2*)P, (const float*)A);
+                         (const float2*)P, (const float*)A, 0);
   });
+  if (!rc && full)
+    rc = dw_timed(4, st, [&] {
+      launch_mirror(full->nx0, full->ny0, full->nz0, a.I, a.ex, a.ey, a.ez, full->u, full->ex,
+                    full->ey, full->ez, 0, nz, st);
+    });
   if (rc) {
     (void)hipFreeAsync(scr, st);
     return rc;
@@ -701,7 +721,8 @@ extern "C" int bpmh_focus_field_debye(int nx0, int ny0, int nz0, double dx, doub
                                       void* stream) {
   if (nx0 < 2 || ny0 < 2 || nz0 < 2 || (nx0 | ny0 | nz0) & 1)
     return bpmh_set_error(BPMH_ERR_UNSUPPORTED, "odd shapes not supported yet");
-  if ((long long)nx0 * ny0 * nz0 > (1LL << 40)) return bpmh_set_error(BPMH_ERR_INVALID, "grid too large");
+  if (ny0 > 65535 || nz0 > 65535 || (long long)nx0 * ny0 > (1LL << 31) - 1)
+    return bpmh_set_error(BPMH_ERR_INVALID, "grid too large");
   DwArgs a;
   int rc = fill_args(a, lam, alphas, n_alphas, int_steps, 1.f, 0.f, BPMH_DW_INDEX, 0.f, 0);
   if (rc) return rc;
@@ -712,9 +733,10 @@ extern "C" int bpmh_focus_field_debye(int nx0, int ny0, int nz0, double dx, doub
   a.x1 = 0.f; a.x2 = (float)(dx * (a.nx - 1.));
   a.y1 = 0.f; a.y2 = (float)(dy * (a.ny - 1.));
   a.z1 = 0.f; a.z2 = (float)(dz * (a.nz - 1.));
-  a.mx0 = nx0; a.my0 = ny0; a.mz0 = nz0;
-  a.ex = (float2*)ex; a.ey = (float2*)ey; a.ez = (float2*)ez; a.I = I;
-  return debye_grid_run(a, int_steps, n_alphas, (hipStream_t)stream);
+  DwFull full;
+  full.nx0 = nx0; full.ny0 = ny0; full.nz0 = nz0;
+  full.u = I; full.ex = (float2*)ex; full.ey = (float2*)ey; full.ez = (float2*)ez;
+  return debye_grid_run(a, int_steps, n_alphas, (hipStream_t)stream, &full);
 }
 
 extern "C" int bpmh_debye_wolf_points(int n, const float* x, const float* y, const float* z,
@@ -750,12 +772,12 @@ extern "C" int bpmh_debye_mirror(int nx0, int ny0, int nz0, const float* u_oct,
     dim3 grid((nx0 / 2 + 127) / 128, ny0, nz0);
     hipLaunchKernelGGL(k_debye_mirror2, grid, dim3(128), 0, (hipStream_t)stream, nx0, ny0, nz0,
                        u_oct, (const float2*)ex_oct, (const float2*)ey_oct, (const float2*)ez_oct,
-                       u, (float2*)ex, (float2*)ey, (float2*)ez);
+                       u, (float2*)ex, (float2*)ey, (float2*)ez, PlaneMap{0, nz0, 0});
   } else {
     dim3 grid((nx0 + 127) / 128, ny0, nz0);
     hipLaunchKernelGGL(k_debye_mirror, grid, dim3(128), 0, (hipStream_t)stream, nx0, ny0, nz0,
                        u_oct, (const float2*)ex_oct, (const float2*)ey_oct, (const float2*)ez_oct,
-                       u, (float2*)ex, (float2*)ey, (float2*)ez);
+                       u, (float2*)ex, (float2*)ey, (float2*)ez, PlaneMap{0, nz0, 0});
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(e));

@@ -56,18 +56,18 @@ __global__ void k_spheres_sum(float* dn, int nx, int ny, int nz, const double* s
   const int j = blockIdx.y;
   const int k = blockIdx.z;
   if (i >= nx) return;
-  const double x = (double)i * dx, y = (double)j * dy, z = (double)k * dz;
+  const double x = d_mul((double)i, dx), y = d_mul((double)j, dy), z = d_mul((double)k, dz);
   float acc = 0.f;
   for (int s = 0; s < n; ++s) {
     const double* p = sph + 5 * s;
-    const double ez = __dsub_rn(z, p[2]);
+    const double ez = d_sub(z, p[2]);
     const double r = p[3];
     if (fabs(ez) >= r) continue;
-    const double ey = __dsub_rn(y, p[1]);
+    const double ey = d_sub(y, p[1]);
     if (fabs(ey) >= r) continue;
-    const double ex = __dsub_rn(x, p[0]);
-    const double d2 = __dadd_rn(__dadd_rn(__dmul_rn(ez, ez), __dmul_rn(ey, ey)), __dmul_rn(ex, ex));
-    if (d2 < __dmul_rn(r, r)) acc += (float)p[4];
+    const double ex = d_sub(x, p[0]);
+    const double d2 = d_add(d_add(d_mul(ez, ez), d_mul(ey, ey)), d_mul(ex, ex));
+    if (d2 < d_mul(r, r)) acc += (float)p[4];
   }
   dn[((size_t)k * ny + j) * nx + i] = acc;
 }

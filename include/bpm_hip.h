@@ -195,14 +195,15 @@ int bpmh_debye_wolf(int nx, int ny, int nz, float x1, float x2, float y1, float 
                     float* ey, float* ez, float* I, void* stream);
 
 /* focus_field_debye in one call: the debye_wolf quadrature on the octant
- * (nz0/2+1, ny0/2+1, nx0/2+1) at x = 0 .. (nx0/2)*dx (likewise y, z), each
- * value written straight to its full-grid (nz0, ny0, nx0) positions
- * f = N0/2 -+ o, the fields conjugated for f_z <= nz0/2 -- the same values
- * as bpmh_debye_wolf on the octant followed by bpmh_debye_mirror, without
- * the octant's round trip through memory.  Replaces the debye_wolf launch
- * and the numpy mirror of bpm/psf/_focus_fields_debye.py:100-143 (x-polarised,
- * Ex0 = 1, Ey0 = 0; lam is the wavelength in the medium, lam/n0).  Even
- * sizes only; outputs (full grid, any may be NULL): ex, ey, ez c64, I f32. */
+ * (nz0/2+1, ny0/2+1, nx0/2+1) at x = 0 .. (nx0/2)*dx (likewise y, z), spread
+ * to the full (nz0, ny0, nx0) grid with the index map of bpmh_debye_mirror
+ * (f -> |f - N0/2|, fields conjugated for f_z <= nz0/2) -- the same bits as
+ * bpmh_debye_wolf on the octant followed by bpmh_debye_mirror, with the
+ * octant in the library's stream-ordered scratch (the pool of
+ * bpmh_debye_wolf's tables).  Replaces the debye_wolf launch and the numpy mirror of
+ * bpm/psf/_focus_fields_debye.py:100-143 (x-polarised, Ex0 = 1, Ey0 = 0; lam
+ * is the wavelength in the medium, lam/n0).  Even sizes only; outputs (full
+ * grid, any may be NULL): ex, ey, ez c64, I f32. */
 int bpmh_focus_field_debye(int nx0, int ny0, int nz0, double dx, double dy, double dz,
                            float lam, const float* alphas, int n_alphas, int int_steps,
                            float* ex, float* ey, float* ez, float* I, void* stream);
@@ -211,9 +212,10 @@ int bpmh_focus_field_debye(int nx0, int ny0, int nz0, double dx, double dy, doub
  * bpmh_focus_field_debye) on the calling thread: with timing on, every
  * kernel is bracketed by a HIP event pair; bpmh_debye_kernel_times waits for
  * them and returns the summed milliseconds and launch counts per kernel
- * kind since the last read-out (0 sample parameters, 1 sample table, 2 phase
- * table, 3 product with its output epilogue).  The reference only prints
- * wall-clock deltas (bpm/psf/_focus_fields_debye.py:100,199,210). */
+ * kind since the last read-out in ms[5], counts[5] (0 sample parameters,
+ * 1 sample table, 2 phase table, 3 product, 4 full-grid spread of
+ * bpmh_focus_field_debye).  The reference only
+ * prints wall-clock deltas (bpm/psf/_focus_fields_debye.py:100,199,210). */
 int bpmh_debye_set_timing(int enable);
 int bpmh_debye_kernel_times(float* ms, int* counts);
 
