@@ -1256,6 +1256,102 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane, c
 }
 
 // ----------------------------------------------------------------------------
+// allocation-free variant of bpmh_propagate
+// ----------------------------------------------------------------------------
+// Its step scratch (weighted-mean and scattering partials, weighted dn0) is
+// this fixed device array instead of plan memory.  Each GPU holds its own
+// copy for the life of the process.  In host code `scratch` names a
+// host-side stand-in, not device memory: the device address comes from
+// hipGetSymbolAddress, once per device, and only that address is used.
+constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[64] = {};  // per device
+
+// Device address of this device's `scratch`, resolved on the first call on
+// the device (with that device current) and kept.
+static int noalloc_scratch(void** out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return bpmh_set_error(BPMH_ERR_UNSUPPORTED, "device index above 63");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      char b[256];
+      snprintf(b, sizeof b, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+               hipGetErrorString(e));
+      return bpmh_set_error(BPMH_ERR_HIP, b);
+    }
+    g_noalloc_ptr[dev] = p;
+  }
+  *out = g_noalloc_ptr[dev];
+  return BPMH_OK;
+}
+
+extern "C" int bpmh_propagate_noalloc(bpmh_plan plan, int batch, int nz, float* plane,
+                                      const void* dn, int dn_kind, long long dn_batch_stride,
+                                      long long dn_numel, int mean_mode, const float* H,
+                                      const float* n_eff, const float* dn0, float n0, float k0,
+                                      float dx, float dy, float dz, float unit_k, int absorb,
+                                      float* out_field, float* out_intens, long long out_numel,
+                                      float* out_sum, const float* scat_w, const float* g_w,
+                                      const float* plain, float* out_scat, void* stream) {
+  PropCall c;
+  c.batch = batch;
+  c.nz = nz;
+  c.plane = (float2*)plane;
+  c.dn = dn;
+  c.dn_kind = dn_kind;
+  c.dn_bstride = dn_batch_stride;
+  c.dn_numel = dn_numel;
+  c.mean_mode = mean_mode;
+  c.H = (const float2*)H;
+  c.n_eff = n_eff;
+  c.dn0 = dn0;
+  c.n0 = n0;
+  c.k0 = k0;
+  c.dx = dx;
+  c.dy = dy;
+  c.dz = dz;
+  c.unit_k = unit_k;
+  c.absorb = absorb;
+  c.out_field = (float2*)out_field;
+  c.out_intens = out_intens;
+  c.out_numel = out_numel;
+  c.out_sum = out_sum;
+  c.scat_w = scat_w;
+  c.g_w = g_w;
+  c.plain = (const float2*)plain;
+  c.out_scat = out_scat;
+  int rc = prop_validate(plan, c);
+  if (rc) return rc;
+  if (!plan->tw_x)
+    return bpmh_set_error(BPMH_ERR_INVALID,
+                          "plan has no device tables: create it with a device present");
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev != plan->device) return bpmh_set_error(BPMH_ERR_INVALID, "plan belongs to another device");
+  StepScratch sc = {nullptr, nullptr, nullptr};
+  const size_t sb = prop_scratch_bytes(plan, c);
+  if (sb > kNoAllocScratchBytes) {
+    char b[256];
+    snprintf(b, sizeof b, "this call needs %zu bytes of scratch; the fixed scratch holds %zu "
+             "(fewer items or planes with scattering sums)", sb, kNoAllocScratchBytes);
+    return bpmh_set_error(BPMH_ERR_INVALID, b);
+  }
+  if (sb) {
+    void* p = nullptr;
+    rc = noalloc_scratch(&p);
+    if (rc) return rc;
+    sc = prop_scratch_carve(plan, c, p);
+  }
+  return prop_run(plan, c, sc, (hipStream_t)stream, false);
+}
+
+// ----------------------------------------------------------------------------
 // adjoint (gradient of a loss on the last plane with respect to dn)
 // ----------------------------------------------------------------------------
 // grad[i] += sum_b gitem[b][i] in item order (deterministic); n counts floats

@@ -139,6 +139,27 @@ int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane, const void* 
                    float* out_intens, long long out_numel, float* out_sum, const float* scat_w,
                    const float* g_w, const float* plain, float* out_scat, void* stream);
 
+/* bpmh_propagate without any allocation: the same arguments and kernels
+ * (bit-identical results), eager launches only (graph mode and timing are
+ * ignored), and the step scratch -- weighted-mean and scattering partials --
+ * taken from a fixed 16 MiB device array that each GPU holds for the life of
+ * the process instead of plan memory.  A call whose scratch would exceed it
+ * (it needs 8*batch*nx bytes for the weighted mean, 8*batch*nz*nx for
+ * scattering sums, plus 256) returns BPMH_ERR_INVALID.  The plan's tables
+ * must already be on the device (plans are created there when a device is
+ * present).  Nothing in the array needs clearing between calls; calls that
+ * overlap on one GPU (several streams or threads) share the array and race.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). */
+int bpmh_propagate_noalloc(bpmh_plan plan, int batch, int nz, float* plane, const void* dn,
+                           int dn_kind, long long dn_batch_stride, long long dn_numel,
+                           int mean_mode, const float* H, const float* n_eff, const float* dn0,
+                           float n0, float k0, float dx, float dy, float dz, float unit_k,
+                           int absorb, float* out_field, float* out_intens, long long out_numel,
+                           float* out_sum, const float* scat_w, const float* g_w,
+                           const float* plain, float* out_scat, void* stream);
+
 /* Coated-sphere refractive-index volume (nz, ny, nx) float32: for each voxel
  * at (i dx, j dy, k dz), dmin = distance to the nearest of the n_points
  * centres (device float32 array n x 3, xyz, sorted by z) whose z is within
