@@ -201,10 +201,12 @@ def plan_shape(lib, plan, kernel):
     """Launch shape the plan uses: rows per row work-group or columns per
     column tile, as "lines=<n>" / "cols=<n>"."""
     import ctypes
-    v = [ctypes.c_int(-1) for _ in range(4)]
+    v = [ctypes.c_int(-1) for _ in range(6)]
     if lib.bpmh_plan_get_variant(plan, *[ctypes.byref(x) for x in v]) != 0:
         return "?"
-    return "lines=%d" % v[2].value if kernel == "row_pass" else "cols=%d" % v[3].value
+    if kernel == "row_pass":
+        return "lines=%d" % v[2].value + ("" if v[4].value == 4 else " e=%d" % v[4].value)
+    return "cols=%d" % v[3].value + ("" if v[5].value == 8 else " e=%d" % v[5].value)
 
 
 def measured_traffic(kernel, nx, ny, shape):

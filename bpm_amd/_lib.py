@@ -35,7 +35,7 @@ SIGNATURES = {
     "bpmh_plan_kernel_times": (_I, [_P, _P, _P]),
     "bpmh_plan_set_graph": (_I, [_P, _I]),
     "bpmh_plan_set_variant": (_I, [_P, _I, _I]),
-    "bpmh_plan_get_variant": (_I, [_P, _P, _P, _P, _P]),
+    "bpmh_plan_get_variant": (_I, [_P, _P, _P, _P, _P, _P, _P]),
     "bpmh_propagator_table": (_I, [_I, _I, _I, _D, _D, _D, _D, _D, _P, _P]),
     "bpmh_propagate": (_I, [_P, _I, _I, _P, _P, _I, _LL, _LL, _I, _P, _P, _P, _F, _F, _F, _F,
                             _F, _F, _I, _P, _P, _LL, _P, _P, _P, _P, _P, _P]),

@@ -70,6 +70,15 @@ constexpr int scale_v(int base, int V) { return V == 0 ? base / 2 : (V == 1 ? ba
 // 1024 threads, and at 256^2..512^2 in batches).
 constexpr int row_e(int n) { return n <= 1024 ? 4 : 8; }
 constexpr int col_e(int n) { return n <= 32 ? 4 : 8; }
+// The other radix schedule of a size, which the autotuner times against the
+// default: 8 elements per thread instead of 4 (when the line still holds two
+// radix-8 stages, N >= 64), or 4 instead of 8.  ealt selects it.
+constexpr int alt_e(int e, int n) { return e == 4 ? (n >= 64 ? 8 : 4) : 4; }
+constexpr int row_e_of(int n, int ealt) { return ealt ? alt_e(row_e(n), n) : row_e(n); }
+constexpr int col_e_of(int n, int ealt) { return ealt ? alt_e(col_e(n), n) : col_e(n); }
+// Variant numbers: row variant = V + 3*ealt (V = 0, 1, 2), column variant =
+// V + 5*ealt (V = 0..4).
+constexpr int kRowShapes = 3, kColShapes = 5;
 
 
 template <int N, int E_>
@@ -89,9 +98,9 @@ struct Base {
   static constexpr int LDS_LINE = COL_LINE * 8;        // bytes per column line
 };
 
-template <int N, int V>
-struct RowCfg : Base<N, row_e(N)> {
-  using Base<N, row_e(N)>::T;
+template <int N, int V, int E_ = row_e(N)>
+struct RowCfg : Base<N, E_> {
+  using Base<N, E_>::T;
   static constexpr int L0 = clampi(256 / T, 1, 16);
   // at least one full wavefront per work-group (the cross-lane reductions
   // use all 64 lanes), at most 16 lines and 1024 threads
@@ -101,23 +110,27 @@ struct RowCfg : Base<N, row_e(N)> {
   static constexpr int THREADS = LINES * T;
 };
 
-template <int N, int V>
-struct ColCfg : Base<N, col_e(N)> {
-  using Base<N, col_e(N)>::T;
-  using Base<N, col_e(N)>::LDS_LINE;
+template <int N, int V, int E_ = col_e(N)>
+struct ColCfg : Base<N, E_> {
+  using Base<N, E_>::T;
+  using Base<N, E_>::LDS_LINE;
   static constexpr int C0 = N >= 4096 ? 1 : clampi(512 / T, 2, 16);
   static constexpr int CMAX0 = 131072 / LDS_LINE < 1 ? 1 : 131072 / LDS_LINE;
   // <= 512 threads with 8 elements per thread (1024 spilled), <= 1024 with 4
-  static constexpr int TMAX = (Base<N, col_e(N)>::E <= 4 ? 1024 : 512) / T < 1
-                                  ? 1 : (Base<N, col_e(N)>::E <= 4 ? 1024 : 512) / T;
+  static constexpr int TMAX = (E_ <= 4 ? 1024 : 512) / T < 1 ? 1 : (E_ <= 4 ? 1024 : 512) / T;
   static constexpr int CMAX = pow2floor(clampi(CMAX0 < TMAX ? CMAX0 : TMAX, 1, 16));
   static constexpr int CMIN = 64 / T < 1 ? 1 : 64 / T;  // >= one full wavefront
   // V = 3: the work-group of V = 0 with P = 2 adjacent columns per thread
   // (tiles twice as wide), so a row of a column pair is one 16-byte load and
-  // store.  Used for columns of 256-1024 points (where it was measured);
-  // elsewhere it is V = 0.
-  static constexpr int P = (V == 3 && N >= 256 && N <= 1024) ? 2 : 1;
-  static constexpr int LC = clampi(scale_v(C0, V == 3 ? 0 : V), CMIN, CMAX);  // lane columns
+  // store.  V = 4: the same with each column's loads and stores issued
+  // separately (8-byte accesses), the first column's loads first, so its
+  // transform can start while the second column's loads are in flight.
+  // Used for columns of 256-1024 points (where they were measured);
+  // elsewhere they are V = 0.
+  static constexpr bool PAIRED = (V == 3 || V == 4) && N >= 256 && N <= 1024;
+  static constexpr int P = PAIRED ? 2 : 1;
+  static constexpr bool SPLIT = PAIRED && V == 4;
+  static constexpr int LC = clampi(scale_v(C0, V >= 3 ? 0 : V), CMIN, CMAX);  // lane columns
   static constexpr int COLS = LC * P;
   static constexpr int THREADS = LC * T;
 };
@@ -232,13 +245,14 @@ struct RowArgs {
   float* gitem;            // (batch, ny, nx) per-item gradient of dn[z] (2 floats per voxel for complex dn)
 };
 
-template <int NX, int V>
-__global__ void __launch_bounds__((RowCfg<NX, V>::THREADS))
+template <int NX, int V, int EE>
+__global__ void __launch_bounds__((RowCfg<NX, V, EE>::THREADS))
 k_row(RowArgs a) {
-  constexpr int E = RowCfg<NX, V>::E, T = RowCfg<NX, V>::T, LINES = RowCfg<NX, V>::LINES;
+  using Cfg = RowCfg<NX, V, EE>;
+  constexpr int E = Cfg::E, T = Cfg::T, LINES = Cfg::LINES;
   // a line of at most 64 threads lies inside one wavefront (T divides 64)
   constexpr bool WAVE = T <= 64;
-  __shared__ float2 lds[LINES][RowCfg<NX, V>::ROW_LINE];
+  __shared__ float2 lds[LINES][Cfg::ROW_LINE];
   const int line = threadIdx.x / T;
   const int t = threadIdx.x - line * T;
   const int row = blockIdx.x * LINES + line;  // in [0, batch*ny)
@@ -377,13 +391,15 @@ struct ColArgs {
 
 // PHASE 0: FFT_y, H, IFFT_y.  PHASE 1: FFT_y, weighted partials, store.
 // PHASE 2: weighted dn0, H(n0+dn0), IFFT_y.
-template <int NY, int V, int PHASE>
-__global__ void __launch_bounds__((ColCfg<NY, V>::THREADS))
+template <int NY, int V, int PHASE, int EE>
+__global__ void __launch_bounds__((ColCfg<NY, V, EE>::THREADS))
 k_col(ColArgs a) {
-  constexpr int E = ColCfg<NY, V>::E, T = ColCfg<NY, V>::T, COLS = ColCfg<NY, V>::COLS;
+  using Cfg = ColCfg<NY, V, EE>;
+  constexpr int E = Cfg::E, T = Cfg::T, COLS = Cfg::COLS;
   // each thread holds P adjacent columns; the LC lane columns of a 16-lane
   // group are the LDS lines it touches together
-  constexpr int P = ColCfg<NY, V>::P, LC = ColCfg<NY, V>::LC;
+  constexpr int P = Cfg::P, LC = Cfg::LC;
+  constexpr bool SPLIT = Cfg::SPLIT;
   // swizzled lines 16/LC float2 apart, or padded lines 9 float2 longer
   // than padded: either way the LC lines a 16-lane group touches together
   // start in different LDS banks.  Two lane columns swizzle when each
@@ -391,9 +407,9 @@ k_col(ColArgs a) {
   // and pad otherwise (the swizzle was slower there).
   constexpr bool SWZ = LC >= 4 || (P == 2 && LC >= 2);
   static_assert(COLS <= 16, "at most 16 columns per tile");
-  constexpr int LINE = SWZ ? NY + 16 / LC : ColCfg<NY, V>::COL_LINE;
+  constexpr int LINE = SWZ ? NY + 16 / LC : Cfg::COL_LINE;
   __shared__ float2 lds[COLS][LINE];
-  using CL = std::conditional_t<SWZ, SwzLayout<4 - ilog2c(LC)>, PadLayout<ColCfg<NY, V>::COL_SH>>;
+  using CL = std::conditional_t<SWZ, SwzLayout<4 - ilog2c(LC)>, PadLayout<Cfg::COL_SH>>;
   const int ntiles = a.nx / COLS;
   const int bx = blockIdx.x;
   // adjacent tiles share cache lines: keep them on one XCD (blocks are
@@ -406,21 +422,32 @@ k_col(ColArgs a) {
   float2* p = a.plane + (size_t)b * NY * a.nx + x0;
 
   // all global loads first (plane, twiddles, the propagator quadrant); with
-  // P = 2 the host guarantees a 16-byte aligned plane (x0 and nx are even)
+  // P = 2 and 16-byte accesses the host guarantees a 16-byte aligned plane
+  // (x0 and nx are even)
   float2 v[P][E];
+  Twiddles<NY, E> tw;
+  // split columns: the twiddles first, so waiting for the first column's
+  // data (loads complete in issue order) does not wait for the second's
+  if constexpr (SPLIT) tw.load(t, a.tw);
+  if constexpr (SPLIT) {
 #pragma unroll
-  for (int m = 0; m < E; ++m) {
-    float2* r = p + (size_t)(t + m * T) * a.nx;
-    if constexpr (P == 2) {
-      const float4 w = *reinterpret_cast<const float4*>(r);
-      v[0][m] = make_float2(w.x, w.y);
-      v[1][m] = make_float2(w.z, w.w);
-    } else {
-      v[0][m] = *r;
+    for (int q = 0; q < P; ++q)
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[q][m] = p[(size_t)(t + m * T) * a.nx + q];
+  } else {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      float2* r = p + (size_t)(t + m * T) * a.nx;
+      if constexpr (P == 2) {
+        const float4 w = *reinterpret_cast<const float4*>(r);
+        v[0][m] = make_float2(w.x, w.y);
+        v[1][m] = make_float2(w.z, w.w);
+      } else {
+        v[0][m] = *r;
+      }
     }
   }
-  Twiddles<NY, E> tw;
-  tw.load(t, a.tw);
+  if constexpr (!SPLIT) tw.load(t, a.tw);
   // H depends on kx^2 and ky^2 only, so H[j][i] = H[(ny-j)%ny][(nx-i)%nx]
   // holds bit for bit; reading the quadrant j<=ny/2, i<=nx/2 keeps the
   // table's footprint at a quarter (L2-resident per XCD strip).
@@ -502,18 +529,27 @@ k_col(ColArgs a) {
       }
     }
   }
-  if constexpr (PHASE != 1) {
+  if constexpr (SPLIT) {
+    // each column stored as soon as its inverse transform is done
 #pragma unroll
-    for (int q = 0; q < P; ++q) line_fft<NY, E, true, CL>(v[q], lds[q * LC + c], t, tw);
-  }
-
+    for (int q = 0; q < P; ++q) {
+      if constexpr (PHASE != 1) line_fft<NY, E, true, CL>(v[q], lds[q * LC + c], t, tw);
 #pragma unroll
-  for (int m = 0; m < E; ++m) {
-    float2* r = p + (size_t)(t + m * T) * a.nx;
-    if constexpr (P == 2)
-      *reinterpret_cast<float4*>(r) = make_float4(v[0][m].x, v[0][m].y, v[1][m].x, v[1][m].y);
-    else
-      *r = v[0][m];
+      for (int m = 0; m < E; ++m) p[(size_t)(t + m * T) * a.nx + q] = v[q][m];
+    }
+  } else {
+    if constexpr (PHASE != 1) {
+#pragma unroll
+      for (int q = 0; q < P; ++q) line_fft<NY, E, true, CL>(v[q], lds[q * LC + c], t, tw);
+    }
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      float2* r = p + (size_t)(t + m * T) * a.nx;
+      if constexpr (P == 2)
+        *reinterpret_cast<float4*>(r) = make_float4(v[0][m].x, v[0][m].y, v[1][m].x, v[1][m].y);
+      else
+        *r = v[0][m];
+    }
   }
 }
 
@@ -586,9 +622,11 @@ constexpr int kGraphCache = 4;
 
 struct bpmh_plan_s {
   int nx, ny;
-  std::vector<float2> htw_x, htw_y;  // stage-major twiddles (host)
-  float2* tw_x;                      // their device copies (first propagation)
-  float2* tw_y;
+  // stage-major twiddles of both radix schedules (index ealt), on the host
+  // and their device copies
+  std::vector<float2> htw_x[2], htw_y[2];
+  float2* tw_x[2];
+  float2* tw_y[2];
   void* work;                        // step scratch (allocated on demand)
   size_t work_bytes;
   int device;
@@ -719,10 +757,11 @@ extern "C" int bpmh_plan_create(int nx, int ny, bpmh_plan* out) {
   p->cap_stream = nullptr;
   p->graph_clock = 0;
   p->device = -1;
-  p->tw_x = nullptr;
-  p->tw_y = nullptr;
-  make_stage_twiddles(nx, row_e(nx), p->htw_x);
-  make_stage_twiddles(ny, col_e(ny), p->htw_y);
+  for (int e = 0; e < 2; ++e) {
+    p->tw_x[e] = p->tw_y[e] = nullptr;
+    make_stage_twiddles(nx, row_e_of(nx, e), p->htw_x[e]);
+    make_stage_twiddles(ny, col_e_of(ny, e), p->htw_y[e]);
+  }
   // device copies now when a device is present (on a machine without one
   // the plan stays host-only: its argument checks still run)
   int ndev = 0;
@@ -743,31 +782,38 @@ extern "C" int bpmh_plan_create(int nx, int ny, bpmh_plan* out) {
 // the plan's device (made at plan creation, or at the first propagation of a
 // plan created without a device).
 static int plan_upload(bpmh_plan p) {
-  if (p->tw_x) return BPMH_OK;
+  if (p->tw_x[0]) return BPMH_OK;
   HIP_TRY(hipGetDevice(&p->device));
-  if (hipMalloc(&p->tw_x, p->htw_x.size() * sizeof(float2)) != hipSuccess ||
-      hipMalloc(&p->tw_y, p->htw_y.size() * sizeof(float2)) != hipSuccess) {
-    (void)hipFree(p->tw_x);
-    (void)hipFree(p->tw_y);
-    p->tw_x = p->tw_y = nullptr;
-    return bpmh_set_error(BPMH_ERR_NOMEM, "hipMalloc failed for twiddles");
-  }
-  if (hipMemcpy(p->tw_x, p->htw_x.data(), p->htw_x.size() * sizeof(float2),
-                hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(p->tw_y, p->htw_y.data(), p->htw_y.size() * sizeof(float2),
-                hipMemcpyHostToDevice) != hipSuccess) {
-    (void)hipFree(p->tw_x);
-    (void)hipFree(p->tw_y);
-    p->tw_x = p->tw_y = nullptr;
-    return bpmh_set_error(BPMH_ERR_HIP, "hipMemcpy failed for twiddles");
+  auto release = [p] {
+    for (int e = 0; e < 2; ++e) {
+      (void)hipFree(p->tw_x[e]);
+      (void)hipFree(p->tw_y[e]);
+      p->tw_x[e] = p->tw_y[e] = nullptr;
+    }
+  };
+  for (int e = 0; e < 2; ++e) {
+    if (hipMalloc(&p->tw_x[e], p->htw_x[e].size() * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&p->tw_y[e], p->htw_y[e].size() * sizeof(float2)) != hipSuccess) {
+      release();
+      return bpmh_set_error(BPMH_ERR_NOMEM, "hipMalloc failed for twiddles");
+    }
+    if (hipMemcpy(p->tw_x[e], p->htw_x[e].data(), p->htw_x[e].size() * sizeof(float2),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->tw_y[e], p->htw_y[e].data(), p->htw_y[e].size() * sizeof(float2),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+      release();
+      return bpmh_set_error(BPMH_ERR_HIP, "hipMemcpy failed for twiddles");
+    }
   }
   return BPMH_OK;
 }
 
 extern "C" int bpmh_plan_destroy(bpmh_plan p) {
   if (!p) return BPMH_OK;
-  (void)hipFree(p->tw_x);
-  (void)hipFree(p->tw_y);
+  for (int e = 0; e < 2; ++e) {
+    (void)hipFree(p->tw_x[e]);
+    (void)hipFree(p->tw_y[e]);
+  }
   if (p->work) (void)hipFree(p->work);
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
   (void)graph_release_all(p);
@@ -857,85 +903,156 @@ extern "C" int bpmh_propagator_table(int nx, int ny, int kind, double dx, double
 typedef void (*row_fn)(const RowArgs&, int batch, hipStream_t);
 typedef void (*col_fn)(const ColArgs&, int batch, hipStream_t);
 
-template <int N, int V>
+template <int N, int V, int E>
 static void launch_row(const RowArgs& a, int batch, hipStream_t s) {
   const int rows = batch * a.ny;
-  hipLaunchKernelGGL((k_row<N, V>), dim3(rows / RowCfg<N, V>::LINES),
-                     dim3(RowCfg<N, V>::THREADS), 0, s, a);
+  hipLaunchKernelGGL((k_row<N, V, E>), dim3(rows / RowCfg<N, V, E>::LINES),
+                     dim3(RowCfg<N, V, E>::THREADS), 0, s, a);
 }
-template <int N, int V, int PHASE>
+template <int N, int V, int PHASE, int E>
 static void launch_col(const ColArgs& a, int batch, hipStream_t s) {
-  hipLaunchKernelGGL((k_col<N, V, PHASE>), dim3(a.nx / ColCfg<N, V>::COLS, batch),
-                     dim3(ColCfg<N, V>::THREADS), 0, s, a);
+  hipLaunchKernelGGL((k_col<N, V, PHASE, E>), dim3(a.nx / ColCfg<N, V, E>::COLS, batch),
+                     dim3(ColCfg<N, V, E>::THREADS), 0, s, a);
 }
 
 #define BPMH_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
 
+// Row launcher of variant v = V + 3*ealt for rows of N points.
+template <int N>
+static row_fn pick_row_n(int v) {
+  constexpr int E0 = row_e_of(N, 0), E1 = row_e_of(N, 1);
+  switch (v) {
+    case 0: return launch_row<N, 0, E0>;
+    case 2: return launch_row<N, 2, E0>;
+    case 3: return launch_row<N, 0, E1>;
+    case 4: return launch_row<N, 1, E1>;
+    case 5: return launch_row<N, 2, E1>;
+    default: return launch_row<N, 1, E0>;
+  }
+}
+
 static row_fn pick_row(int n, int v) {
-#define ROW_CASE(N)                                    \
-  case N:                                              \
-    return v == 0 ? launch_row<N, 0> : (v == 2 ? launch_row<N, 2> : launch_row<N, 1>);
+#define ROW_CASE(N) \
+  case N:           \
+    return pick_row_n<N>(v);
   switch (n) { BPMH_SIZES(ROW_CASE) }
 #undef ROW_CASE
   return nullptr;
 }
 
+// Column launcher of variant v = V + 5*ealt for columns of N points.
+template <int N, int PHASE>
+static col_fn pick_col_n(int v) {
+  constexpr int E0 = col_e_of(N, 0), E1 = col_e_of(N, 1);
+  switch (v) {
+    case 0: return launch_col<N, 0, PHASE, E0>;
+    case 2: return launch_col<N, 2, PHASE, E0>;
+    case 3: return launch_col<N, 3, PHASE, E0>;
+    case 4: return launch_col<N, 4, PHASE, E0>;
+    case 5: return launch_col<N, 0, PHASE, E1>;
+    case 6: return launch_col<N, 1, PHASE, E1>;
+    case 7: return launch_col<N, 2, PHASE, E1>;
+    case 8: return launch_col<N, 3, PHASE, E1>;
+    case 9: return launch_col<N, 4, PHASE, E1>;
+    default: return launch_col<N, 1, PHASE, E0>;
+  }
+}
+
 template <int PHASE>
 static col_fn pick_col(int n, int v) {
-#define COL_CASE(N)                                                              \
-  case N:                                                                        \
-    return v == 0 ? launch_col<N, 0, PHASE>                                      \
-                  : (v == 2 ? launch_col<N, 2, PHASE>                            \
-                            : (v == 3 ? launch_col<N, 3, PHASE> : launch_col<N, 1, PHASE>));
+#define COL_CASE(N) \
+  case N:           \
+    return pick_col_n<N, PHASE>(v);
   switch (n) { BPMH_SIZES(COL_CASE) }
 #undef COL_CASE
   return nullptr;
 }
 
+template <int N>
+static int col_cols_n(int v) {
+  constexpr int E0 = col_e_of(N, 0), E1 = col_e_of(N, 1);
+  switch (v) {
+    case 0: return ColCfg<N, 0, E0>::COLS;
+    case 2: return ColCfg<N, 2, E0>::COLS;
+    case 3: return ColCfg<N, 3, E0>::COLS;
+    case 4: return ColCfg<N, 4, E0>::COLS;
+    case 5: return ColCfg<N, 0, E1>::COLS;
+    case 6: return ColCfg<N, 1, E1>::COLS;
+    case 7: return ColCfg<N, 2, E1>::COLS;
+    case 8: return ColCfg<N, 3, E1>::COLS;
+    case 9: return ColCfg<N, 4, E1>::COLS;
+    default: return ColCfg<N, 1, E0>::COLS;
+  }
+}
+
+template <int N>
+static int row_lines_n(int v) {
+  constexpr int E0 = row_e_of(N, 0), E1 = row_e_of(N, 1);
+  switch (v) {
+    case 0: return RowCfg<N, 0, E0>::LINES;
+    case 2: return RowCfg<N, 2, E0>::LINES;
+    case 3: return RowCfg<N, 0, E1>::LINES;
+    case 4: return RowCfg<N, 1, E1>::LINES;
+    case 5: return RowCfg<N, 2, E1>::LINES;
+    default: return RowCfg<N, 1, E0>::LINES;
+  }
+}
+
 static int col_tiles(int nx, int ny, int v) {
-#define TILE_CASE(N)                                                              \
-  case N:                                                                         \
-    return nx / (v == 0 ? ColCfg<N, 0>::COLS                                      \
-                        : (v == 2 ? ColCfg<N, 2>::COLS                            \
-                                  : (v == 3 ? ColCfg<N, 3>::COLS : ColCfg<N, 1>::COLS)));
+#define TILE_CASE(N) \
+  case N:            \
+    return nx / col_cols_n<N>(v);
   switch (ny) { BPMH_SIZES(TILE_CASE) }
 #undef TILE_CASE
   return 0;
 }
 
-// Column variant 3 loads and stores 16 bytes per column pair: a plane that
-// is only 8-byte aligned (a view at an odd element offset) runs variant 0
-// (one column per thread), which gives the same bits.
+// Column variants with V = 3 load and store 16 bytes per column pair: a
+// plane that is only 8-byte aligned (a view at an odd element offset) runs
+// V = 0 of the same radix schedule (one column per thread), which gives the
+// same bits.
 static int col_variant_for(int cv, const float2* plane) {
-  return (cv == 3 && (reinterpret_cast<uintptr_t>(plane) & 15)) ? 0 : cv;
+  return (cv % kColShapes == 3 && (reinterpret_cast<uintptr_t>(plane) & 15)) ? cv - 3 : cv;
 }
 
-static void variant_shape(int nx, int ny, int rv, int cv, int* lines, int* cols) {
-#define SHAPE_CASE(N)                                                                   \
-  case N:                                                                               \
-    *lines = rv == 0 ? RowCfg<N, 0>::LINES : (rv == 2 ? RowCfg<N, 2>::LINES : RowCfg<N, 1>::LINES); \
+static void variant_shape(int nx, int ny, int rv, int cv, int* lines, int* cols, int* row_e,
+                          int* col_e) {
+#define SHAPE_CASE(N)                 \
+  case N:                             \
+    *lines = row_lines_n<N>(rv);      \
+    *row_e = row_e_of(N, rv / kRowShapes); \
     break;
   switch (nx) { BPMH_SIZES(SHAPE_CASE) }
 #undef SHAPE_CASE
+#define CE_CASE(N)                    \
+  case N:                             \
+    *col_e = col_e_of(N, cv / kColShapes); \
+    break;
+  switch (ny) { BPMH_SIZES(CE_CASE) }
+#undef CE_CASE
   *cols = nx / col_tiles(nx, ny, cv);
 }
 
 extern "C" int bpmh_plan_set_variant(bpmh_plan p, int row_variant, int col_variant) {
   if (!p) return bpmh_set_error(BPMH_ERR_INVALID, "null plan");
-  if (row_variant < 0 || row_variant > 2 || col_variant < 0 || col_variant > 3)
-    return bpmh_set_error(BPMH_ERR_INVALID, "row variants are 0, 1 or 2; column variants 0 to 3");
+  if (row_variant < 0 || row_variant >= 2 * kRowShapes || col_variant < 0 ||
+      col_variant >= 2 * kColShapes)
+    return bpmh_set_error(BPMH_ERR_INVALID, "row variants are 0 to 5; column variants 0 to 9");
   p->row_v = row_variant;
   p->col_v = col_variant;
   return BPMH_OK;
 }
 
 extern "C" int bpmh_plan_get_variant(bpmh_plan p, int* row_variant, int* col_variant,
-                                     int* lines_per_group, int* cols_per_group) {
-  if (!p || !row_variant || !col_variant || !lines_per_group || !cols_per_group)
+                                     int* lines_per_group, int* cols_per_group, int* row_elems,
+                                     int* col_elems) {
+  if (!p || !row_variant || !col_variant || !lines_per_group || !cols_per_group || !row_elems ||
+      !col_elems)
     return bpmh_set_error(BPMH_ERR_INVALID, "null argument");
   *row_variant = p->row_v;
   *col_variant = p->col_v;
-  variant_shape(p->nx, p->ny, p->row_v, p->col_v, lines_per_group, cols_per_group);
+  variant_shape(p->nx, p->ny, p->row_v, p->col_v, lines_per_group, cols_per_group, row_elems,
+                col_elems);
   return BPMH_OK;
 }
 
@@ -1050,7 +1167,7 @@ static int prop_run(bpmh_plan plan, const PropCall& c, const StepScratch& sc, hi
   RowArgs ra;
   memset(&ra, 0, sizeof ra);
   ra.plane = c.plane;
-  ra.tw = plan->tw_x;
+  ra.tw = plan->tw_x[plan->row_v / kRowShapes];
   ra.dn = c.dn;
   ra.dn_bstride = c.dn_bstride;
   ra.dn_kind = c.dn_kind;
@@ -1069,7 +1186,7 @@ static int prop_run(bpmh_plan plan, const PropCall& c, const StepScratch& sc, hi
   ColArgs ca;
   memset(&ca, 0, sizeof ca);
   ca.plane = c.plane;
-  ca.tw = plan->tw_y;
+  ca.tw = plan->tw_y[cv / kColShapes];
   ca.nx = nx;
   ca.nz = nz;
   ca.hmode = c.mean_mode;  // 0 table, 1 n_eff array, 2 weighted
@@ -1328,7 +1445,7 @@ extern "C" int bpmh_propagate_noalloc(bpmh_plan plan, int batch, int nz, float* 
   c.out_scat = out_scat;
   int rc = prop_validate(plan, c);
   if (rc) return rc;
-  if (!plan->tw_x)
+  if (!plan->tw_x[0])
     return bpmh_set_error(BPMH_ERR_INVALID,
                           "plan has no device tables: create it with a device present");
   int dev = -1;
@@ -1400,13 +1517,14 @@ extern "C" int bpmh_adjoint_segment(bpmh_plan plan, int batch, int z_lo, int z_h
   if (urc) return urc;
   hipStream_t s = (hipStream_t)stream;
   const row_fn row = pick_row(nx, plan->row_v);
-  const col_fn col0 = pick_col<0>(ny, col_variant_for(plan->col_v, g));
+  const int cv = col_variant_for(plan->col_v, g);
+  const col_fn col0 = pick_col<0>(ny, cv);
   const int comps = dn_kind == BPMH_DN_C64 ? 2 : 1;  // floats of gradient per voxel
 
   RowArgs ra;
   memset(&ra, 0, sizeof ra);
   ra.plane = g;
-  ra.tw = plan->tw_x;
+  ra.tw = plan->tw_x[plan->row_v / kRowShapes];
   ra.dn = dn;
   ra.dn_kind = dn_kind;
   ra.ny = ny;
@@ -1422,7 +1540,7 @@ extern "C" int bpmh_adjoint_segment(bpmh_plan plan, int batch, int z_lo, int z_h
   ColArgs ca;
   memset(&ca, 0, sizeof ca);
   ca.plane = g;
-  ca.tw = plan->tw_y;
+  ca.tw = plan->tw_y[cv / kColShapes];
   ca.nx = nx;
   ca.hmode = 0;
   ca.H = H;

@@ -1,8 +1,11 @@
-"""Autotuner for the launch shapes of the split-step passes.
+"""Autotuner for the launch shapes and radix schedules of the split-step passes.
 
 For a grid (Nx, Ny) and batch size it times every (row, column) variant of
-``bpmh_plan_set_variant`` on the device (HIP events on the current stream
-around a short propagation) and keeps the fastest.  Results are cached in
+``bpmh_plan_set_variant`` -- work-group shapes, paired or split column
+accesses, and for each pass both radix schedules (4 or 8 elements per
+thread, e.g. a 1024-point line as 4.4.4.4.4 or 2.8.8.8) -- on the device
+(HIP events on the current stream around a short propagation) and keeps the
+fastest.  Results are cached in
 ``tuning_cache.json`` next to the library, keyed by "gfx:NxxNy:batch";
 ``get_plan`` applies a cached choice automatically.
 """
@@ -45,15 +48,20 @@ def lookup(nx, ny, batch=1):
     return None
 
 
+ROW_VARIANTS, COL_VARIANTS = 6, 10  # V + 3*s, V + 5*s (s: radix schedule)
+
+
 def shape_of(plan):
-    r, c, l, k = (ctypes.c_int() for _ in range(4))
-    _lib.call("bpmh_plan_get_variant", plan, ctypes.byref(r), ctypes.byref(c), ctypes.byref(l),
-              ctypes.byref(k))
-    return r.value, c.value, l.value, k.value
+    """(row variant, column variant, lines per row work-group, columns per
+    column tile, row elements per thread, column elements per thread)."""
+    v = [ctypes.c_int() for _ in range(6)]
+    _lib.call("bpmh_plan_get_variant", plan, *[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)
 
 
 def autotune(nx, ny, batch=1, nz=65, reps=3, units=(.1, .1, .1), lam=.5, n0=1.33, save=None):
-    """Time all 3x4 (row, column) variants for one grid/batch; returns (best, table)."""
+    """Time all 6x10 (row, column) variants for one grid/batch (variants that
+    map to an identical kernel pair are timed once); returns (best, table)."""
     from . import bpm3d as B
     t = _lib.torch()
     plan = B.get_plan(nx, ny, None)
@@ -64,13 +72,13 @@ def autotune(nx, ny, batch=1, nz=65, reps=3, units=(.1, .1, .1), lam=.5, n0=1.33
     acc = t.empty((batch, ny, nx), dtype=t.float32, device=_lib.device())
     plane = t.empty_like(u0)
     table = {}
-    for rv in range(3):
-        for cv in range(4):
+    for rv in range(ROW_VARIANTS):
+        for cv in range(COL_VARIANTS):
             _lib.call("bpmh_plan_set_variant", plan, rv, cv)
             shp = shape_of(plan)
-            key = [shp[2], shp[3]]
-            if cv == 3 and 256 <= ny <= 1024:
-                key.append(2)  # two columns per thread: a kernel of its own
+            key = [shp[2], shp[3], shp[4], shp[5]]
+            if cv % 5 in (3, 4) and 256 <= ny <= 1024:
+                key.append(cv % 5)  # paired / split columns: kernels of their own
             if key in [v["shape"] for v in table.values()]:
                 continue  # clamped to an identical launch shape
             def run():

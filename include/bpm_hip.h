@@ -81,19 +81,27 @@ int bpmh_plan_kernel_times(bpmh_plan plan, float* ms, int* counts);
  * (bpm/bpm/bpm_3d.py:283-382).  Ignored while timing is on. */
 int bpmh_plan_set_graph(bpmh_plan plan, int enable);
 
-/* Launch-shape variants of the row and column passes (0, 1, 2: half, the
- * default, twice the lines / columns per work-group; column variant 3: the
- * work-group of 0 with two adjacent columns per thread (tiles twice as wide,
- * 16-byte accesses), for
- * columns of 256-1024 points and 16-byte aligned planes, elsewhere the same
- * as 0), chosen per grid size and batch by the autotuner
- * (bpm_amd/tuning.py).  Every variant gives the same bits.  get_variant also
- * reports the lines per row work-group and columns per column tile the
- * variants map to.  No reference counterpart (gputools picks its own FFT
- * launch shapes). */
+/* Launch-shape and radix-schedule variants of the row and column passes,
+ * chosen per grid size and batch by the autotuner (bpm_amd/tuning.py).
+ * Row variant = V + 3*s, column variant = V + 5*s, where s selects the
+ * radix schedule: 0 the size's default elements per thread (rows 4 up to
+ * 1024 points, 8 above; columns 8, 4 up to 32 points), 1 the other one (8
+ * instead of 4 from 64 points on, 4 instead of 8), e.g. a 1024-point line as
+ * 4.4.4.4.4 or 2.8.8.8.  V: 0, 1, 2 half, the default, twice the lines /
+ * columns per work-group; column V = 3 the work-group of 0 with two adjacent
+ * columns per thread (tiles twice as wide, 16-byte accesses; 8-byte aligned
+ * planes run V = 0), V = 4 the same with each column's loads and stores
+ * issued separately (the first column transformed while the second one's
+ * loads are in flight); V = 3, 4 apply to columns of 256-1024 points and are
+ * V = 0 elsewhere.  Variants of one radix schedule give the same bits; the
+ * two schedules round differently (both within the parity bar).
+ * get_variant also reports the lines per row work-group, columns per column
+ * tile and elements per thread of each pass.  No reference counterpart
+ * (gputools picks its own FFT plans, bpm/bpm/bpm_3d.py:188). */
 int bpmh_plan_set_variant(bpmh_plan plan, int row_variant, int col_variant);
 int bpmh_plan_get_variant(bpmh_plan plan, int* row_variant, int* col_variant,
-                          int* lines_per_group, int* cols_per_group);
+                          int* lines_per_group, int* cols_per_group, int* row_elems,
+                          int* col_elems);
 
 /* Fill H (ny, nx) complex64 with the angular-spectrum propagator.
  * kind BPMH_H_F32 replaces the compute_propagator kernel launch

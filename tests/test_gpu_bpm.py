@@ -296,8 +296,8 @@ def test_launch_variants_agree(bpm, nx, ny):
     lib = _lib.load()
     outs = {}
     try:
-        for rv in range(3):
-            for cv in range(4):
+        for rv in range(6):
+            for cv in range(10):
                 _lib.check(lib.bpmh_plan_set_variant(plan, rv, cv))
                 for mode in ("ignore", "weighted"):
                     p = u0.clone()
@@ -305,18 +305,24 @@ def test_launch_variants_agree(bpm, nx, ny):
                                        mean_mode=mode, H=H, tuned=False)
                     outs[(rv, cv, mode)] = p
         # per-line arithmetic and the per-column reductions do not depend on
-        # the launch shape, so every variant gives the same bits
+        # the launch shape, so every variant of one pair of radix schedules
+        # (rv // 3, cv // 5) gives the same bits; the schedules differ only
+        # in rounding
         for (rv, cv, mode), p in outs.items():
-            assert torch.equal(p, outs[(1, 1, mode)]), (rv, cv, mode)
+            ref = outs[(3 * (rv // 3) + 1, 5 * (cv // 5) + 1, mode)]
+            assert torch.equal(p, ref), (rv, cv, mode)
+            d = ((p - outs[(1, 1, mode)]).abs().norm() / outs[(1, 1, mode)].abs().norm()).item()
+            assert d < 1e-5, (rv, cv, mode, d)
     finally:
         _lib.check(lib.bpmh_plan_set_variant(plan, 1, 1))
 
 
 @pytest.mark.parametrize("nx,ny", [(512, 512), (64, 1024)])
 def test_paired_column_variant_on_every_column_path(bpm, nx, ny):
-    """Column variant 3 (two adjacent columns per thread, 16-byte accesses)
-    against variant 0 (the same work-group, one column per thread) on the column
-    pass's other paths: the simple mean (propagator formula with n_eff per
+    """Column variants 3 (two adjacent columns per thread, 16-byte accesses)
+    and 4 (the same with each column's accesses separate) against variant 0
+    (the same work-group, one column per thread) on the column pass's other
+    paths: the simple mean (propagator formula with n_eff per
     plane), the scattering sums, and a plane that is only 8-byte aligned (a
     view at an odd element offset, which the library runs as variant 0).
     Same bits throughout."""
@@ -339,7 +345,7 @@ def test_paired_column_variant_on_every_column_path(bpm, nx, ny):
     lib = _lib.load()
     outs = {}
     try:
-        for cv in (0, 3):
+        for cv in (0, 3, 4):
             _lib.check(lib.bpmh_plan_set_variant(plan, 1, cv))
             p = u0.clone()
             B.propagate_fields(p, Nz, units, LAM, N0, dn=dn, dn_kind=_lib.DN_F32,
@@ -360,6 +366,7 @@ def test_paired_column_variant_on_every_column_path(bpm, nx, ny):
             outs[(cv, "unaligned")] = p.clone()
         for kind in ("simple", "scat", "sums", "unaligned"):
             assert torch.equal(outs[(3, kind)], outs[(0, kind)]), kind
+            assert torch.equal(outs[(4, kind)], outs[(0, kind)]), kind
         assert outs[(0, "sums")].abs().sum() > 0
     finally:
         _lib.check(lib.bpmh_plan_set_variant(plan, 1, 1))

@@ -1,6 +1,6 @@
 """The shipped autotuner cache (bpm_amd/tuning_cache.json) on the CPU: every
-entry names launch-shape variants the C ABI accepts (rows 0-2, columns
-0-3), its best choice is the fastest entry of its own table, and lookup
+entry names variants the C ABI accepts (rows 0-5, columns 0-9: launch
+shape plus radix schedule), its best choice is the fastest entry of its own table, and lookup
 falls back to the nearest cached batch of the same grid."""
 import json
 
@@ -18,14 +18,14 @@ def test_cache_entries_are_valid_and_consistent():
         assert nx >= 16 and ny >= 16 and nx & (nx - 1) == 0 and ny & (ny - 1) == 0
         assert int(batch) >= 1
         rv, cv = entry["best"]
-        assert 0 <= rv <= 2 and 0 <= cv <= 3, key
+        assert 0 <= rv <= 5 and 0 <= cv <= 9, key
         table = entry["table"]
         assert "%d,%d" % (rv, cv) in table, key
         fastest = min(table, key=lambda k: table[k]["ms"])
         assert fastest == "%d,%d" % (rv, cv), key
         for k, v in table.items():
             r, c = (int(x) for x in k.split(","))
-            assert 0 <= r <= 2 and 0 <= c <= 3 and v["ms"] > 0, (key, k)
+            assert 0 <= r <= 5 and 0 <= c <= 9 and v["ms"] > 0, (key, k)
 
 
 def test_lookup_uses_nearest_cached_batch(monkeypatch):
