@@ -169,9 +169,12 @@ def served_extra(seed, n_jobs=200, load=0.8, plain_jobs=48, runs=2):
     from bpm_amd.serving import BpmServer, plain_loop, poisson_jobs
     jobs = poisson_jobs(seed=seed, n_jobs=n_jobs, rate_hz=1.)
     srv = BpmServer(hbm_budget=48 << 30, slots=4)
+    unf = BpmServer(hbm_budget=48 << 30, slots=4, fuse=False)
     try:
         srv.run(jobs[:4], realtime=False)  # warm-up (plans, propagators)
         cap = srv.run(jobs, realtime=False)
+        unf.run(jobs[:4], realtime=False)
+        cap_unfused = unf.run(jobs, realtime=False)
         rate = load * n_jobs / cap["wall_s"]
         rs = []
         for r_i in range(runs):
@@ -182,15 +185,19 @@ def served_extra(seed, n_jobs=200, load=0.8, plain_jobs=48, runs=2):
                   file=sys.stderr, flush=True)
     finally:
         srv.close()
+        unf.close()
     p = plain_loop(jobs[:plain_jobs])
     thr = [r["throughput"] for r in rs]
     return {"workload": "%d jobs, grids {128,256,512,1024}^2, depth U{64..512}, 8-64 angles "
                         "(tilt <= 0.3 rad), spheres as the 256^2 workload; 4 slots, 48 GiB "
                         "HBM budget; latency = arrival -> results on host" % n_jobs,
             "capacity_voxel_steps_per_s": cap["throughput"], "capacity_wall_s": cap["wall_s"],
+            "capacity_jobs_per_launch": float(np.mean(cap["launch_sizes"])),
+            "capacity_unfused_voxel_steps_per_s": cap_unfused["throughput"],
             "poisson_rate_jobs_per_s": rate, "load_of_capacity": load,
             "p50_ms": [r["p50_s"] * 1e3 for r in rs], "p99_ms": [r["p99_s"] * 1e3 for r in rs],
             "served_voxel_steps_per_s": thr, "served_wall_s": [r["wall_s"] for r in rs],
+            "served_jobs_per_launch": [float(np.mean(r["launch_sizes"])) for r in rs],
             "plain_loop_voxel_steps_per_s": p["throughput"], "plain_loop_jobs": plain_jobs,
             "plain_wall_s": p["wall_s"],
             "served_over_plain": [v / p["throughput"] for v in thr],

@@ -41,6 +41,8 @@ SIGNATURES = {
                             _F, _F, _I, _P, _P, _LL, _P, _P, _P, _P, _P, _P]),
     "bpmh_propagate_noalloc": (_I, [_P, _I, _I, _P, _P, _I, _LL, _LL, _I, _P, _P, _P, _F, _F,
                                     _F, _F, _F, _F, _I, _P, _P, _LL, _P, _P, _P, _P, _P, _P]),
+    "bpmh_propagate_groups": (_I, [_P, _I, _P, _P, _P, _P, _I, _LL, _LL, _P, _F, _I, _P, _P,
+                                   _P]),
     "bpmh_adjoint_segment": (_I, [_P, _I, _I, _I, _P, _P, _LL, _P, _I, _LL, _P, _F, _I, _P, _P,
                                   _P]),
     "bpmh_debye_wolf": (_I, [_I, _I, _I, _F, _F, _F, _F, _F, _F, _F, _P, _I, _I, _F, _F, _I,

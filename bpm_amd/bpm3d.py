@@ -13,7 +13,7 @@ import numpy as np
 
 from . import _lib
 
-__all__ = ["bpm_3d", "propagate_fields", "get_plan", "Plan"]
+__all__ = ["bpm_3d", "propagate_fields", "propagate_groups", "get_plan", "Plan"]
 
 _MEAN = {"ignore": _lib.MEAN_IGNORE, "simple": _lib.MEAN_SIMPLE, "weighted": _lib.MEAN_WEIGHTED}
 _plans = {}
@@ -204,6 +204,50 @@ def propagate_fields(plane, nz, units, lam, n0=1., dn=None, dn_kind=None, dn_bat
               float(np.float32(dz)), float(np.float32(k0 * dz)), int(absorbing_width),
               _lib.ptr(out_field), _lib.ptr(out_intens), int(min(outs) if outs else 0),
               _lib.ptr(out_sum), _lib.ptr(sw), _lib.ptr(gw), _lib.ptr(plain), _lib.ptr(sout),
+              _lib.stream_handle())
+    return plane
+
+
+def propagate_groups(plane, groups, units, lam, dn, H, dn_group_stride, out_sum=None,
+                     absorbing_width=0, plan=None, done_events=None):
+    """Ragged batch on the device (static propagator H, mean_mode "ignore"):
+    ``groups`` = [(items, nz), ...] in order of non-increasing nz; group g's
+    ``items`` fields (consecutive in ``plane``, (sum items, Ny, Nx) complex64)
+    propagate through ``nz`` planes of dn volume g, at element offset
+    g*dn_group_stride of the float32 tensor ``dn``.  ``out_sum``: None or
+    (sum items, Ny, Nx) float32 sums of |u|^2.  ``done_events``: None or one
+    torch.cuda.Event per group, recorded as soon as that group is final.
+    Every item gives the bits of propagate_fields on its own volume with the
+    same plan variant."""
+    import torch as t
+    tot = sum(int(g[0]) for g in groups)
+    Bn, Ny, Nx = plane.shape
+    if Bn != tot or plane.dtype != t.complex64 or not plane.is_contiguous() or not plane.is_cuda:
+        raise ValueError("plane must be a contiguous (sum items, Ny, Nx) complex64 GPU tensor")
+    if dn.dtype != t.float32 or not dn.is_contiguous() or dn.device != plane.device:
+        raise ValueError("dn must be a contiguous float32 tensor on the plane's device")
+    if out_sum is not None and (out_sum.dtype != t.float32 or not out_sum.is_contiguous()
+                                or out_sum.numel() < tot * Ny * Nx or out_sum.device != plane.device):
+        raise ValueError("out_sum must be a contiguous float32 tensor of (sum items, Ny, Nx)")
+    if H is None or H.dtype != t.complex64 or H.numel() < Ny * Nx or H.device != plane.device:
+        raise ValueError("H must be an (Ny, Nx) complex64 tensor on the plane's device")
+    n = len(groups)
+    items = (ctypes.c_int * n)(*[int(g[0]) for g in groups])
+    nzs = (ctypes.c_int * n)(*[int(g[1]) for g in groups])
+    evs = None
+    if done_events is not None:
+        if len(done_events) != n:
+            raise ValueError("one event per group")
+        for e in done_events:
+            if not e.cuda_event:
+                e.record()  # torch creates the event at its first record
+        evs = (ctypes.c_void_p * n)(*[ctypes.c_void_p(e.cuda_event) for e in done_events])
+    if plan is None:
+        plan = get_plan(Nx, Ny, tot)
+    k0 = 2. * np.pi / lam
+    _lib.call("bpmh_propagate_groups", plan, n, items, nzs, _lib.ptr(plane), _lib.ptr(dn),
+              _lib.DN_F32, int(dn_group_stride), int(dn.numel()), _lib.ptr(H),
+              float(np.float32(k0 * units[2])), int(absorbing_width), _lib.ptr(out_sum), evs,
               _lib.stream_handle())
     return plane
 

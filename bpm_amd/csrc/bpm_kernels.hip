@@ -79,6 +79,7 @@ constexpr int col_e_of(int n, int ealt) { return ealt ? alt_e(col_e(n), n) : col
 // Variant numbers: row variant = V + 3*ealt (V = 0, 1, 2), column variant =
 // V + 5*ealt (V = 0..4).
 constexpr int kRowShapes = 3, kColShapes = 5;
+#define BPMH_MAX_GROUPS 16
 
 
 template <int N, int E_>
@@ -219,7 +220,11 @@ struct RowArgs {
   float2* plane;           // (rows = batch*ny, nx)
   const float2* tw;        // stage-major twiddles of the nx-point transform
   const void* dn;          // (nz, ny, nx) per batch item, kind dn_kind
-  long long dn_bstride;    // elements between batch items' dn (0: shared)
+  long long dn_bstride;    // elements between batch items' (or groups') dn (0: shared)
+  // ragged batch: items [gend[g-1], gend[g]) read dn volume g; ngroups = 0:
+  // item b reads volume b
+  int ngroups;
+  int gend[BPMH_MAX_GROUPS];
   int dn_kind;
   int ny, nz, z;
   float unit_k;            // float32(k0*dz)
@@ -234,7 +239,7 @@ struct RowArgs {
   int acc_init;
   int inverse;             // input is F_x u: inverse FFT first
   int screen;              // apply dn[z] screen
-  int forward;             // forward FFT_x before storing
+  int fwd_items;           // items b < fwd_items get the forward FFT_x before storing
   // adjoint step: v is the gradient G_z (torch convention, 2 dL/du*); the
   // row pass writes unit_k*Im(conj(G_z) u_z) per item and multiplies by the
   // conjugate screen
@@ -244,6 +249,15 @@ struct RowArgs {
   int useg_z;              // plane of z inside the segment
   float* gitem;            // (batch, ny, nx) per-item gradient of dn[z] (2 floats per voxel for complex dn)
 };
+
+// dn volume of item b: b itself, or in a ragged batch the group holding b
+// (a few scalar compares; b is uniform over a work-group).
+BPMH_DEV int dn_volume(const RowArgs& a, int b) {
+  if (a.ngroups == 0) return b;
+  int g = 0;
+  while (g + 1 < a.ngroups && b >= a.gend[g]) ++g;
+  return g;
+}
 
 template <int NX, int V, int EE>
 __global__ void __launch_bounds__((RowCfg<NX, V, EE>::THREADS))
@@ -267,10 +281,11 @@ k_row(RowArgs a) {
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = ld_stream(p + t + m * T);
   Twiddles<NX, E> tw;
-  if (a.inverse || a.forward) tw.load(t, a.tw);
+  const bool forward = b < a.fwd_items;  // uniform: a work-group's rows are one item's
+  if (a.inverse || forward) tw.load(t, a.tw);
   float2 d[E];
   if (a.screen) {
-    const size_t slice = ((size_t)a.z * a.ny + y) * NX + (size_t)b * a.dn_bstride;
+    const size_t slice = ((size_t)a.z * a.ny + y) * NX + (size_t)dn_volume(a, b) * a.dn_bstride;
 #pragma unroll
     for (int m = 0; m < E; ++m) d[m] = load_dn(a.dn, a.dn_kind, slice + t + m * T);
   }
@@ -361,7 +376,7 @@ k_row(RowArgs a) {
     }
   }
 
-  if (a.forward) line_fft<NX, E, false, SwzLayout<4>, WAVE>(v, lds[line], t, tw);
+  if (forward) line_fft<NX, E, false, SwzLayout<4>, WAVE>(v, lds[line], t, tw);
 #pragma unroll
   for (int m = 0; m < E; ++m) st_stream(p + t + m * T, v[m]);
 }
@@ -1257,7 +1272,7 @@ static int prop_run(bpmh_plan plan, const PropCall& c, const StepScratch& sc, hi
   ra.acc_init = 1;
   ra.inverse = 0;
   ra.screen = 0;
-  ra.forward = nz > 1;
+  ra.fwd_items = nz > 1 ? batch : 0;
   timed(TK_ROW, [&] { row(ra, batch, s); });
   ra.acc_init = 0;
   ra.inverse = 1;
@@ -1271,7 +1286,7 @@ static int prop_run(bpmh_plan plan, const PropCall& c, const StepScratch& sc, hi
       timed(TK_COL, [&] { col0(ca, batch, s); });
     }
     ra.z = z;
-    ra.forward = z < nz - 1;
+    ra.fwd_items = z < nz - 1 ? batch : 0;
     timed(TK_ROW, [&] { row(ra, batch, s); });
   }
   if (c.out_scat) {
@@ -1370,6 +1385,116 @@ extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane, c
     sc = prop_scratch_carve(plan, c, scr);
   }
   return prop_run(plan, c, sc, s, true);
+}
+
+// ----------------------------------------------------------------------------
+// ragged batch: groups of fields with their own dn volumes and depths
+// ----------------------------------------------------------------------------
+extern "C" int bpmh_propagate_groups(bpmh_plan plan, int n_groups, const int* items,
+                                     const int* nzs, float* plane_, const void* dn, int dn_kind,
+                                     long long dn_group_stride, long long dn_numel,
+                                     const float* H_, float unit_k, int absorb, float* out_sum,
+                                     void** group_done, void* stream) {
+  float2* plane = (float2*)plane_;
+  const float2* H = (const float2*)H_;
+  char b[256];
+  if (!plan) return bpmh_set_error(BPMH_ERR_INVALID, "null plan");
+  if (!plane || !H) return bpmh_set_error(BPMH_ERR_INVALID, "null plane or propagator");
+  if (n_groups < 1 || n_groups > BPMH_MAX_GROUPS || !items || !nzs)
+    return bpmh_set_error(BPMH_ERR_INVALID, "1 to 16 groups with item and depth arrays");
+  if (dn_kind != BPMH_DN_NONE && dn_kind != BPMH_DN_F32 && dn_kind != BPMH_DN_F16 &&
+      dn_kind != BPMH_DN_C64)
+    return bpmh_set_error(BPMH_ERR_INVALID, "unknown dn kind");
+  if (dn_kind != BPMH_DN_NONE && !dn) return bpmh_set_error(BPMH_ERR_INVALID, "null dn");
+  if (dn_group_stride < 0 || absorb < 0)
+    return bpmh_set_error(BPMH_ERR_INVALID, "negative stride or absorbing width");
+  const int nx = plan->nx, ny = plan->ny;
+  const long long n = (long long)nx * ny;
+  int total = 0;
+  for (int g = 0; g < n_groups; ++g) {
+    if (items[g] < 1 || nzs[g] < 1)
+      return bpmh_set_error(BPMH_ERR_INVALID, "every group needs >= 1 item and >= 1 plane");
+    if (g > 0 && nzs[g] > nzs[g - 1])
+      return bpmh_set_error(BPMH_ERR_INVALID, "groups must come in order of non-increasing depth");
+    if (dn_kind != BPMH_DN_NONE &&
+        (long long)g * dn_group_stride + (long long)nzs[g] * n > dn_numel) {
+      snprintf(b, sizeof b, "dn holds %lld elements; group %d reads %d planes at offset %lld",
+               dn_numel, g, nzs[g], (long long)g * dn_group_stride);
+      return bpmh_set_error(BPMH_ERR_INVALID, b);
+    }
+    total += items[g];
+    if (total > 65535) return bpmh_set_error(BPMH_ERR_INVALID, "batch too large");
+  }
+  if ((long long)total * ny > (1LL << 30)) return bpmh_set_error(BPMH_ERR_INVALID, "batch too large");
+  int rc = plan_upload(plan);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const row_fn row = pick_row(nx, plan->row_v);
+  const int cv = col_variant_for(plan->col_v, plane);
+  const col_fn col = pick_col<0>(ny, cv);
+
+  RowArgs ra;
+  memset(&ra, 0, sizeof ra);
+  ra.plane = plane;
+  ra.tw = plan->tw_x[plan->row_v / kRowShapes];
+  ra.dn = dn;
+  ra.dn_bstride = dn_group_stride;
+  ra.dn_kind = dn_kind;
+  ra.ngroups = n_groups;
+  for (int g = 0, e = 0; g < n_groups; ++g) ra.gend[g] = (e += items[g]);
+  ra.ny = ny;
+  ra.nz = nzs[0];
+  ra.unit_k = unit_k;
+  ra.inv_n = 1.0f / ((float)nx * (float)ny);
+  ra.absorb = absorb;
+  ra.acc = out_sum;
+
+  ColArgs ca;
+  memset(&ca, 0, sizeof ca);
+  ca.plane = plane;
+  ca.tw = plan->tw_y[cv / kColShapes];
+  ca.nx = nx;
+  ca.nz = nzs[0];
+  ca.hmode = 0;
+  ca.H = H;
+
+  // items still propagating at plane z (and, for the forward transform, past
+  // it): a prefix, the groups being in order of non-increasing depth
+  auto items_deeper = [&](int z) {
+    int c = 0;
+    for (int g = 0; g < n_groups; ++g)
+      if (nzs[g] > z) c += items[g];
+    return c;
+  };
+  auto mark_done = [&](int z) -> int {  // groups whose last plane is z
+    if (!group_done) return BPMH_OK;
+    for (int g = 0; g < n_groups; ++g)
+      if (nzs[g] - 1 == z && group_done[g]) HIP_TRY(hipEventRecord((hipEvent_t)group_done[g], s));
+    return BPMH_OK;
+  };
+  ra.z = 0;
+  ra.acc_init = 1;
+  ra.inverse = 0;
+  ra.screen = 0;
+  ra.fwd_items = items_deeper(1);
+  row(ra, total, s);
+  rc = mark_done(0);
+  if (rc) return rc;
+  ra.acc_init = 0;
+  ra.inverse = 1;
+  ra.screen = dn_kind != BPMH_DN_NONE;
+  for (int z = 1; z < nzs[0]; ++z) {
+    const int active = items_deeper(z);
+    ca.z = z;
+    col(ca, active, s);
+    ra.z = z;
+    ra.fwd_items = items_deeper(z + 1);
+    row(ra, active, s);
+    rc = mark_done(z);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipGetLastError());
+  return BPMH_OK;
 }
 
 // ----------------------------------------------------------------------------
@@ -1552,7 +1677,7 @@ extern "C" int bpmh_adjoint_segment(bpmh_plan plan, int batch, int z_lo, int z_h
     ra.useg_z = z - z_lo;
     ra.inverse = z != z_hi;
     ra.screen = 1;
-    ra.forward = 1;
+    ra.fwd_items = batch;
     row(ra, batch, s);
     hipLaunchKernelGGL(k_grad_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, gitem,
                        grad_dn + (size_t)z * n, batch, n);
@@ -1562,7 +1687,7 @@ extern "C" int bpmh_adjoint_segment(bpmh_plan plan, int batch, int z_lo, int z_h
   ra.adjoint = 0;
   ra.screen = 0;
   ra.inverse = 1;
-  ra.forward = 0;
+  ra.fwd_items = 0;
   row(ra, batch, s);
   HIP_TRY(hipGetLastError());
   return BPMH_OK;

@@ -40,8 +40,9 @@ def bpm_3d_spheres(size, units, lam=.5, u0=None, points=None, dn_inner=0.0, rad_
                   return_scattering=return_scattering, use_fresnel_approx=use_fresnel_approx)
 
 
-def spheres_dn_device(size, units, centres, radii, dns):
-    """(Nz,Ny,Nx) float32 GPU tensor, sum of dn_s inside each sphere (strict)."""
+def spheres_dn_device(size, units, centres, radii, dns, out=None):
+    """(Nz,Ny,Nx) float32 GPU tensor, sum of dn_s inside each sphere (strict);
+    written into ``out`` (a contiguous (Nz,Ny,Nx) float32 GPU tensor) if given."""
     Nx, Ny, Nz = size
     dx, dy, dz = units
     tab = np.column_stack([np.asarray(centres, np.float64).reshape(-1, 3),
@@ -49,7 +50,13 @@ def spheres_dn_device(size, units, centres, radii, dns):
                            np.asarray(dns, np.float64).ravel()])
     t = _lib.torch()
     dev_tab = t.from_numpy(np.ascontiguousarray(tab).ravel()).to(_lib.device())
-    dn = _lib.empty((Nz, Ny, Nx), np.float32)
+    if out is None:
+        dn = _lib.empty((Nz, Ny, Nx), np.float32)
+    else:
+        if (tuple(out.shape) != (Nz, Ny, Nx) or out.dtype != t.float32 or not out.is_contiguous()
+                or not out.is_cuda):
+            raise ValueError("out must be a contiguous (Nz,Ny,Nx) float32 GPU tensor")
+        dn = out
     _lib.call("bpmh_spheres_dn", int(Nx), int(Ny), int(Nz), float(dx), float(dy), float(dz),
               _lib.ptr(dev_tab), int(tab.shape[0]), _lib.ptr(dn), _lib.stream_handle())
     return dn

@@ -168,6 +168,26 @@ int bpmh_propagate_noalloc(bpmh_plan plan, int batch, int nz, float* plane, cons
                            float* out_sum, const float* scat_w, const float* g_w,
                            const float* plain, float* out_scat, void* stream);
 
+/* Ragged batch on the static-propagator (Bpm3d / mean_mode ignore) path:
+ * n_groups (1-16) groups, group g holding items[g] fields that propagate
+ * through their own dn volume, nzs[g] planes at dn + g*dn_group_stride
+ * elements, with the groups in order of non-increasing depth.  The fields
+ * are consecutive in plane (sum(items), ny, nx) c64 (u0 in, each field's
+ * last plane out) and out_sum (sum(items), ny, nx) f32 or NULL (sum of
+ * |u|^2 over the group's planes).  Every step runs only the items still
+ * propagating (a prefix of them), so jobs of one grid size but different
+ * depths and volumes share one launch sequence; each item gives the bits of
+ * bpmh_propagate on its own volume (same plan variant).  group_done: NULL
+ * or n_groups HIP events (hipEvent_t) to record on `stream` as soon as that
+ * group's planes and sums are final.  dn_numel: elements dn holds (every
+ * group's planes must lie inside).  No reference counterpart (the reference
+ * propagates one field per call, bpm/bpm/bpm_3d.py:39-101); it serves
+ * several requests of one grid size in one launch sequence. */
+int bpmh_propagate_groups(bpmh_plan plan, int n_groups, const int* items, const int* nzs,
+                          float* plane, const void* dn, int dn_kind, long long dn_group_stride,
+                          long long dn_numel, const float* H, float unit_k, int absorb,
+                          float* out_sum, void** group_done, void* stream);
+
 /* Coated-sphere refractive-index volume (nz, ny, nx) float32: for each voxel
  * at (i dx, j dy, k dz), dmin = distance to the nearest of the n_points
  * centres (device float32 array n x 3, xyz, sorted by z) whose z is within

@@ -153,3 +153,33 @@ def test_library_checks_extents_before_launch(lib):
         assert rc == _lib.ERR_UNSUPPORTED
     finally:
         assert lib.bpmh_plan_destroy(h) == 0
+
+
+def test_ragged_batch_arguments_checked_before_launch(lib):
+    """bpmh_propagate_groups refuses groups out of depth order, a dn volume
+    too short for some group, empty groups and more than 16 groups, before
+    touching a device (plan creation is host-only here)."""
+    h = ctypes.c_void_p()
+    assert lib.bpmh_plan_create(32, 32, ctypes.byref(h)) == 0
+    try:
+        n = 32 * 32
+        fake = ctypes.c_void_p(4096)
+
+        def call(items, nzs, stride, numel):
+            k = len(items)
+            it = (ctypes.c_int * k)(*items)
+            nz = (ctypes.c_int * k)(*nzs)
+            return lib.bpmh_propagate_groups(h, k, it, nz, fake, fake, _lib.DN_F32, stride, numel,
+                                             fake, 1.25, 0, None, None, None)
+
+        cases = [
+            (lambda: call([2, 3], [10, 20], 20 * n, 40 * n), b"non-increasing depth"),
+            (lambda: call([2, 3], [20, 10], 20 * n, 29 * n), b"group 1 reads"),
+            (lambda: call([2, 0], [20, 10], 20 * n, 40 * n), b">= 1 item"),
+            (lambda: call([1] * 17, [5] * 17, 5 * n, 17 * 5 * n), b"1 to 16 groups"),
+        ]
+        for fn, msg in cases:
+            rc = fn()
+            assert rc == _lib.ERR_INVALID and msg in lib.bpmh_last_error(), (rc, msg)
+    finally:
+        assert lib.bpmh_plan_destroy(h) == 0

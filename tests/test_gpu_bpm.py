@@ -481,3 +481,51 @@ def test_weighted_mode_small_workload_vs_c64_and_f64_oracles(bpm):
     say("   sum |u|^2  rel-L2 vs c64 oracle %.2e, vs f64 oracle %.2e" % (errs["c64"][1], errs["f64"][1]))
     assert errs["c64"][0] < TOL and errs["c64"][1] < TOL
     assert errs["f64"][0] < 1e-3 and errs["f64"][1] < 1e-3
+
+
+def test_ragged_batch_equals_per_group_runs(bpm):
+    """bpmh_propagate_groups: three groups of 3, 1 and 2 fields with their own
+    dn volumes and depths 40, 25 and 25 in one launch sequence; every field's
+    exit plane and summed intensity equal its group's own propagate_fields
+    run bit for bit (same plan variant), one field matches the complex64
+    oracle of the class path, and each group's event is recorded."""
+    import torch
+    from bpm_amd import bpm3d as B
+    from bpm_amd import _lib
+    say("case ragged batch")
+    Nx, Ny, units = 256, 128, (.1, .1, .1)
+    groups = [(3, 40), (1, 25), (2, 25)]
+    nzmax = 40
+    rng = np.random.default_rng(41)
+    vols = [rng_dn((nzmax, Ny, Nx), "f32", seed=50 + g) for g in range(len(groups))]
+    dn = torch.from_numpy(np.stack(vols)).cuda()
+    tot = sum(g[0] for g in groups)
+    u0 = np.exp(1j * rng.uniform(0, 2, (tot, Ny, Nx))).astype(np.complex64)
+    H = B.propagator_table(Nx, Ny, units, LAM, N0, _lib.H_F64)
+    plan = B.Plan(Nx, Ny)
+    try:
+        _lib.check(_lib.load().bpmh_plan_set_variant(plan, 1, 1))
+        plane = torch.from_numpy(u0).cuda()
+        acc = torch.empty((tot, Ny, Nx), dtype=torch.float32, device="cuda")
+        evs = [torch.cuda.Event() for _ in groups]
+        B.propagate_groups(plane, groups, units, LAM, dn, H, nzmax * Ny * Nx, out_sum=acc,
+                           plan=plan, done_events=evs)
+        torch.cuda.synchronize()
+        assert all(e.query() for e in evs)
+        i0 = 0
+        for g, (items, nz) in enumerate(groups):
+            p1 = torch.from_numpy(u0[i0:i0 + items]).cuda()
+            a1 = torch.empty((items, Ny, Nx), dtype=torch.float32, device="cuda")
+            B.propagate_fields(p1, nz, units, LAM, N0, dn=dn[g, :nz].contiguous(),
+                               dn_kind=_lib.DN_F32, mean_mode="ignore", H=H, out_sum=a1,
+                               plan=plan, tuned=False)
+            assert torch.equal(p1, plane[i0:i0 + items]), g
+            assert torch.equal(a1, acc[i0:i0 + items]), g
+            i0 += items
+        ref = O.propagate_single((Nx, Ny, 25), units, LAM, N0, vols[1][:25], u0[3],
+                                 return_full=False)
+        e = O.rel_l2(plane[3].cpu().numpy(), ref)
+        say("   group 1 vs c64 oracle rel-L2 %.2e" % e)
+        assert e < TOL
+    finally:
+        plan.close()

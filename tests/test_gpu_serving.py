@@ -1,10 +1,12 @@
 """GPU tests of the continuous-batching server (bpm_amd.serving).
 
 Every job's results (B exit fields and summed intensities) must be the same
-bits whether the job is served alone or in a mix of overlapping jobs, and the
-same as the per-request loop that propagates each angle alone; one angle per
-grid size is checked against the complex64 oracle (bar 1e-4 rel-L2) and the
-float64 oracle (error stated).  The per-request entry each job reproduces is
+bits whether the job is served alone or in a mix of overlapping and fused
+jobs (the server keeps one launch shape and radix schedule per grid size),
+and agree with the per-request loop that propagates each angle alone (which
+uses its own batch-1 tuning, possibly another radix schedule) to the parity
+bar, 1e-4 rel-L2 (measured up to 1.3e-5 over ~500 planes); one angle per grid size is checked against the complex64 oracle (bar
+1e-4 rel-L2) and the float64 oracle (error stated).  The per-request entry each job reproduces is
 bpm/bpm/bpm_3d.py:39-101 (with the class path's static propagator,
 bpm/bpmclass/_bpm3d_ocl.py:66-134).
 """
@@ -35,24 +37,41 @@ def srv_mod():
     return serving
 
 
+def rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
 def test_served_results_equal_alone_and_in_mix(srv_mod):
-    # small jobs: continuous batching with more jobs than slots, each job's
-    # fields equal to the per-request baseline (each angle alone) bit for bit
+    # small jobs: continuous batching with more jobs than slots, fused and
+    # unfused; each job's fields equal to its run alone bit for bit and to
+    # the per-request baseline (each angle alone) to the parity bar
     S = srv_mod
     say("case serving small")
-    jobs = S.poisson_jobs(seed=3, n_jobs=6, rate_hz=200., grids=(64, 128, 256), depth=(16, 48),
+    jobs = S.poisson_jobs(seed=3, n_jobs=8, rate_hz=200., grids=(64, 128, 256), depth=(16, 48),
                           angles=(2, 6))
-    budget = 2 * max(S.job_footprint(j) for j in jobs)
-    srv = S.BpmServer(hbm_budget=budget, slots=3)
-    try:
-        r = srv.run(jobs, realtime=True, keep_results=True)
-    finally:
-        srv.close()
+    budget = 3 * max(S.job_footprint(j) for j in jobs)
+    out = {}
+    for fuse in (True, False):
+        srv = S.BpmServer(hbm_budget=budget, slots=3, fuse=fuse)
+        try:
+            r = srv.run(jobs, realtime=True, keep_results=True)
+            alone = {j.jid: srv.run([j], realtime=False, keep_results=True)["results"][j.jid]
+                     for j in jobs}
+        finally:
+            srv.close()
+        say("   fuse=%s: launches %s, p50 %.1f ms p99 %.1f ms"
+            % (fuse, r["launch_sizes"], r["p50_s"] * 1e3, r["p99_s"] * 1e3))
+        for j in jobs:
+            assert all(np.array_equal(a, b) for a, b in zip(r["results"][j.jid], alone[j.jid])), j.jid
+        out[fuse] = r
+        assert r["p99_s"] >= r["p50_s"] > 0
+    assert max(out[True]["launch_sizes"]) > 1  # some jobs were fused
     p = S.plain_loop(jobs, keep_results=True)
     for j in jobs:
-        gu, gi = r["results"][j.jid]
-        pu, pi = p["results"][j.jid]
-        assert np.array_equal(gu, pu) and np.array_equal(gi, pi), j.jid
+        for fuse in (True, False):
+            gu, gi = out[fuse]["results"][j.jid]
+            pu, pi = p["results"][j.jid]
+            assert rel(gu, pu) < 1e-4 and rel(gi, pi) < 1e-4, (j.jid, fuse)
     say("   p50 %.1f ms p99 %.1f ms" % (r["p50_s"] * 1e3, r["p99_s"] * 1e3))
     assert r["p99_s"] >= r["p50_s"] > 0
     # a job that does not fit the budget is refused up front
@@ -100,13 +119,15 @@ def test_c4_served_stream_alone_mix_and_oracle(srv_mod):
         au, ai = alone[j.jid]
         assert mu.shape == (len(j.angles), j.N, j.N) and mi.shape == mu.shape
         assert np.array_equal(mu, au) and np.array_equal(mi, ai), j.jid
-    say("   every job: mix == alone bit for bit")
+    say("   every job: mix == alone bit for bit; launches %s" % mix["launch_sizes"])
     plain = S.plain_loop(jobs, keep_results=True)
+    worst = 0.
     for j in jobs:
         pu, pi = plain["results"][j.jid]
         mu, mi = mix["results"][j.jid]
-        assert np.array_equal(mu, pu) and np.array_equal(mi, pi), j.jid
-    say("   every job: served == per-request loop (each angle alone) bit for bit")
+        worst = max(worst, rel(mu, pu), rel(mi, pi))
+    say("   served vs per-request loop (each angle alone): max rel-L2 %.2e" % worst)
+    assert worst < 1e-4
     # one angle of one job per grid size against the oracles
     O.set_fft_workers(min(16, len(os.sched_getaffinity(0))))
     try:
@@ -114,7 +135,7 @@ def test_c4_served_stream_alone_mix_and_oracle(srv_mod):
             j = min((j for j in jobs if j.N == N), key=lambda j: j.nz)
             b = len(j.angles) // 2
             dn = spheres_dn((N, N, j.nz), UNITS, *S.job_spheres(j, UNITS))
-            u0 = S.job_u0(j, UNITS, LAM, N0)[b]
+            u0 = S.job_u0(j, UNITS, LAM, N0)[b].cpu().numpy()  # the served input
             mu, mi = mix["results"][j.jid]
             t0 = time.perf_counter()
             errs = {}
