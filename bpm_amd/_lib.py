@@ -133,6 +133,8 @@ def to_device(x, np_dtype):
     if isinstance(x, t.Tensor):
         return x.to(device=device(), dtype=tdt).contiguous()
     a = np.ascontiguousarray(np.asarray(x).astype(np_dtype, copy=False))
+    if not a.flags.writeable:  # torch.from_numpy wants a writable array
+        a = a.copy()
     return t.from_numpy(a).to(device())
 
 
