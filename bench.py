@@ -400,7 +400,9 @@ def floor_extra(torch, dn_g):
       copies), 32 B per voxel-step;
     * "elementwise_pair": a plane copy (16 B/voxel) and a plane update with
       the dn slice (plane read and write + dn read, 20 B/voxel), 36 B per
-      voxel-step, as torch element-wise kernels."""
+      voxel-step, as torch element-wise kernels;
+    * "pass_pattern_batch1/16": the passes' own access patterns and launch
+      shapes without the transforms (pass_pattern_floor)."""
     nz, ny, nx = dn_g.shape
     plane = torch.zeros((ny, nx), dtype=torch.complex64, device="cuda")
     tmp = torch.empty_like(plane)
@@ -443,7 +445,58 @@ def floor_extra(torch, dn_g):
                      "us_per_step": ms * 1e3 / (nz - 1), "voxel_steps_per_s": vs,
                      "effective_GBps": nbytes * vs / 1e9,
                      "hbm_frac": nbytes * vs / 1e9 / HBM_PEAK_GBS}
+    out.update(pass_pattern_floor(torch, dn_g))
     return out
+
+
+def pass_pattern_floor(torch, dn_g, batches=(1, 16), nz_batched=129):
+    """bpmh_floor_steps: per step the column pass's and the row pass's global
+    loads and stores in the autotuned launch shapes of each batch size, with
+    no transforms (36 B per voxel-step), graph-replayed like the z-loop.  The
+    FFT passes' step time over this one's is the cost of the transforms on
+    top of the step's data movement (its cross-XCD exchange included)."""
+    from bpm_amd import _lib
+    from bpm_amd.bpm3d import Plan, apply_tuning
+    nz, ny, nx = dn_g.shape
+    res = {}
+    for B in batches:
+        depth = nz if B == 1 else min(nz, nz_batched)
+        plan = Plan(nx, ny)
+        apply_tuning(plan, nx, ny, B)
+        plane = torch.zeros((B, ny, nx), dtype=torch.complex64, device="cuda")
+
+        def run():
+            _lib.call("bpmh_floor_steps", plan, int(B), int(depth), _lib.ptr(plane),
+                      _lib.ptr(dn_g), int(dn_g.numel()), _lib.stream_handle())
+
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            run()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            run()
+        g.replay()
+        torch.cuda.synchronize()
+        reps = 5 if B == 1 else 3
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(reps):
+            g.replay()
+        ev1.record()
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / reps
+        del g
+        plan.close()
+        vs = B * nx * ny * (depth - 1) / (ms * 1e-3)
+        res["pass_pattern_batch%d" % B] = {
+            "bytes_per_voxel_step": 36.0, "planes": depth,
+            "us_per_field_step": ms * 1e3 / (depth - 1) / B, "voxel_steps_per_s": vs,
+            "effective_GBps": 36. * vs / 1e9, "hbm_frac": 36. * vs / 1e9 / HBM_PEAK_GBS}
+    return res
 
 
 def differentiable_extra(seed):

@@ -17,8 +17,20 @@ it is a ``torch.autograd.Function``:
   u_z), returned as the complex gradient dL/dRe + i dL/dIm (torch's
   convention for complex leaves).
 
-Memory: checkpoints B*ceil(nz/K) planes + one segment of B*(K+1) planes.
+With ``mean_mode="weighted"`` (the default of ``bpm_3d``,
+bpm/bpm/bpm_3d.py:149-159,302-311) the propagator of each step is H(n0+dn0)
+with dn0 = sum|U| d / sum|U| over the k-space plane U of that step (d = dn
+slice 0 at the same flat index), and the screen uses dn - dn0.  dn0 depends
+on the field and on slice 0, so the adjoint (``bpmh_weighted_segment``)
+carries two more terms per step: dL/ddn0 = -sum(screen gradient) + the
+propagator's derivative, summed over k-space, and through it a k-space term
+in the field gradient and the gradient of slice 0.  Real dn only.
+
+Memory: checkpoints B*ceil(nz/K) planes + one segment of B*(K+1) planes
+(weighted: about 2*B*(K+2) planes of recorded forward state).
 """
+import ctypes
+
 import numpy as np
 
 from . import _lib
@@ -79,16 +91,79 @@ def _make_fn():
     return _Fn
 
 
+def _weighted_args(units, lam, n0):
+    dx, dy, dz = units
+    k0 = 2. * np.pi / lam
+    f = lambda v: float(np.float32(v))
+    return f(n0), f(k0), f(dx), f(dy), f(dz), f(k0 * dz)
+
+
+def _weighted_call(plane, z0, z1, dnc, consts, absorb, g, grad_dn, work):
+    n0, k0, dx, dy, dz, unit_k = consts
+    _lib.call("bpmh_weighted_segment", B.get_plan(dnc.shape[2], dnc.shape[1]), int(plane.shape[0]),
+              int(z0), int(z1), _lib.ptr(plane), _lib.ptr(dnc), int(dnc.numel()), n0, k0, dx, dy,
+              dz, unit_k, int(absorb), _lib.ptr(g) if g is not None else None,
+              _lib.ptr(grad_dn) if grad_dn is not None else None, _lib.ptr(work),
+              int(work.numel()), _lib.stream_handle())
+
+
+def _work(nx, ny, batch, z0, z1, adjoint):
+    t = _lib.torch()
+    n = ctypes.c_longlong()
+    _lib.call("bpmh_weighted_segment_work", B.get_plan(nx, ny), int(batch), int(z0), int(z1),
+              int(adjoint), ctypes.byref(n))
+    return t.empty(int(n.value), dtype=t.float32, device=_lib.device())
+
+
+def _make_weighted_fn():
+    t = _lib.torch()
+
+    class _WFn(t.autograd.Function):
+        @staticmethod
+        def forward(ctx, dn, u0, units, lam, n0, absorbing_width, checkpoint):
+            nz, ny, nx = dn.shape
+            Bn = u0.shape[0]
+            consts = _weighted_args(units, lam, n0)
+            plane = u0.detach().to(dtype=t.complex64).contiguous().clone()
+            dnc = dn.detach().contiguous()
+            segs = _segments(nz, checkpoint)
+            work = _work(nx, ny, Bn, 0, 0, False)
+            ckpts = []
+            for z0, z1 in segs:
+                ckpts.append(plane.clone())
+                _weighted_call(plane, z0, z1, dnc, consts, absorbing_width, None, None, work)
+            ctx.save_for_backward(dnc, *ckpts)
+            ctx.meta = (consts, absorbing_width, segs)
+            return plane
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            dnc, *ckpts = ctx.saved_tensors
+            consts, absorb, segs = ctx.meta
+            nz, ny, nx = dnc.shape
+            Bn = grad_out.shape[0]
+            g = grad_out.to(dtype=t.complex64).contiguous().clone()
+            grad_dn = t.zeros_like(dnc)
+            longest = max(z1 - z0 for z0, z1 in segs)
+            work = _work(nx, ny, Bn, 0, longest, True)
+            for (z0, z1), ck in zip(reversed(segs), reversed(ckpts)):
+                _weighted_call(ck, z0, z1, dnc, consts, absorb, g, grad_dn, work)
+            return grad_dn, g, None, None, None, None, None
+
+    return _WFn
+
+
 _FN = None
+_WFN = None
 
 
-def propagate(dn, u0, units=(.1, .1, .1), lam=.5, n0=1., absorbing_width=0, checkpoint=16):
+def propagate(dn, u0, units=(.1, .1, .1), lam=.5, n0=1., absorbing_width=0, checkpoint=16,
+              mean_mode="ignore"):
     """Exit fields (B, Ny, Nx) complex64 of u0 (B, Ny, Nx) through dn
     (Nz, Ny, Nx), float32 or complex64 (both GPU tensors); differentiable in
-    dn and u0."""
-    global _FN
-    if _FN is None:
-        _FN = _make_fn()
+    dn and u0.  mean_mode "ignore" is the Bpm3d path (static propagator);
+    "weighted" is bpm_3d's default (float32 dn)."""
+    global _FN, _WFN
     t = _lib.torch()
     if dn.dtype not in (t.float32, t.complex64):
         raise ValueError("dn must be float32 or complex64")
@@ -96,5 +171,15 @@ def propagate(dn, u0, units=(.1, .1, .1), lam=.5, n0=1., absorbing_width=0, chec
         raise ValueError("dn must be (Nz,Ny,Nx) and u0 (B,Ny,Nx)")
     if checkpoint < 1:
         raise ValueError("checkpoint must be >= 1")
+    if mean_mode == "weighted":
+        if dn.dtype != t.float32:
+            raise NotImplementedError("the weighted-mean adjoint takes float32 dn")
+        if _WFN is None:
+            _WFN = _make_weighted_fn()
+        return _WFN.apply(dn, u0, tuple(units), lam, n0, int(absorbing_width), int(checkpoint))
+    if mean_mode != "ignore":
+        raise NotImplementedError("mean_mode must be 'ignore' or 'weighted'")
+    if _FN is None:
+        _FN = _make_fn()
     return _FN.apply(dn, u0, tuple(units), lam, n0, int(absorbing_width), int(checkpoint))
 

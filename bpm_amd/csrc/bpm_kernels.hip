@@ -168,6 +168,17 @@ BPMH_DEV float2 prop_f32(float kx2, float ky2, float a, float dz) {
 
 BPMH_DEV float n_term_f32(float n, float k0) { return f_mul(f_mul(f_mul(n, n), k0), k0); }
 
+// prop_f32 (the same operations, so the same H) that also returns
+// beta = sqrt(a - kx^2 - ky^2), 0 where the component does not propagate
+// (H = 0 there, and its derivative with respect to dn0 is taken as 0).
+BPMH_DEV float2 prop_f32_beta(float kx2, float ky2, float a, float dz, float* beta) {
+  const float tmp = f_sub(f_sub(a, kx2), ky2);
+  const float h0 = tmp < 0.f ? 0.f : __fsqrt_rn(tmp);
+  const float2 cs = sincos_f32_via_f64(f_mul(-dz, h0));
+  *beta = tmp > 0.f ? h0 : 0.f;
+  return tmp >= 0.f ? cs : make_float2(0.f, 0.f);
+}
+
 // Weighted mean (bpm_3d.py:149-159,302-303): dn0 = dn_sum/u_sum over the nx
 // per-column partials of item b (x = dn part, y = u part) that column pass A
 // wrote.  Column pass B forms it once per work-group: lane p of the first
@@ -176,6 +187,7 @@ BPMH_DEV float n_term_f32(float n, float k0) { return f_mul(f_mul(f_mul(n, n), k
 // broadcast through LDS, and the work-group of tile 0 stores it for the row
 // pass.  The kernel boundary after pass A orders the partials' stores before
 // these loads.
+// slot[0] receives dn0, slot[1] the weight sum sum|U| (the adjoint needs it).
 BPMH_DEV float weighted_dn0_group(const float2* __restrict__ wp, int nparts, float* slot) {
   if (threadIdx.x < 64) {
     const int lane = threadIdx.x;
@@ -190,10 +202,38 @@ BPMH_DEV float weighted_dn0_group(const float2* __restrict__ wp, int nparts, flo
       sd += __shfl_xor(sd, o, 64);
       su += __shfl_xor(su, o, 64);
     }
-    if (lane == 0) *slot = sd / su;
+    if (lane == 0) {
+      slot[0] = sd / su;
+      slot[1] = su;
+    }
   }
   __syncthreads();
-  return *slot;
+  return slot[0];
+}
+
+// Sum of n1 floats at p1 and of n2 floats at p2 by the first wavefront in a
+// fixed order (lane l takes elements l, l+64, ..., then an xor butterfly),
+// so every work-group of an item gets the same bits; returned to all
+// threads through slot[0..1].
+BPMH_DEV float2 group_sum_pair(const float* __restrict__ p1, int n1, const float* __restrict__ p2,
+                               int n2, float* slot) {
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    float s1 = 0.f, s2 = 0.f;
+    for (int i = lane; i < n1; i += 64) s1 += p1[i];
+    for (int i = lane; i < n2; i += 64) s2 += p2[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      s1 += __shfl_xor(s1, o, 64);
+      s2 += __shfl_xor(s2, o, 64);
+    }
+    if (lane == 0) {
+      slot[0] = s1;
+      slot[1] = s2;
+    }
+  }
+  __syncthreads();
+  return make_float2(slot[0], slot[1]);
 }
 
 // Sum over the T threads of each column of a tile, in a fixed binary-tree
@@ -402,15 +442,38 @@ struct ColArgs {
   const float2* pw;        // plane-wave DC per plane (nz)
   float2* spart;           // scattering per-column partials (batch, nz, nx)
   int conj_h;              // adjoint: multiply by conj(H)
+  // weighted forward, recording for the adjoint (null: nothing recorded):
+  // PHASE 1 also stores U = F u of this step (item stride kout_bstride),
+  // PHASE 2 stores this step's dn0 and sum|U| per item
+  float2* kout;
+  long long kout_bstride;
+  float* rec_c;
+  float* rec_w;
+  // weighted adjoint (PHASE 3, 4) of the step whose forward k-space plane is
+  // U (uk, item stride uk_bstride) with dn0 c = cz[b] and sum|U| = wz[b]
+  const float2* uk;
+  long long uk_bstride;
+  const float* cz;
+  const float* wz;
+  float* s2part;           // (batch, nx): per column sum Im(conj(X) H U) / beta
+  const float* s1part;     // (batch, ny): per row sum of the screen gradient
+  float* gd_item;          // (batch, ny, nx): += dL/d(weight slice) per item
 };
 
 // PHASE 0: FFT_y, H, IFFT_y.  PHASE 1: FFT_y, weighted partials, store.
 // PHASE 2: weighted dn0, H(n0+dn0), IFFT_y.
+// Adjoint of a weighted step (X = F G_v, the k-space gradient):
+// PHASE 3: FFT_y, per-column partials of sum Im(conj(X) H U)/beta, store X.
+// PHASE 4: dL/ddn0 from the partials, conj(H) X plus the term of dn0's
+//          dependence on |U|, the weight-slice gradient, IFFT_y.
 template <int NY, int V, int PHASE, int EE>
 __global__ void __launch_bounds__((ColCfg<NY, V, EE>::THREADS))
 k_col(ColArgs a) {
   using Cfg = ColCfg<NY, V, EE>;
   constexpr int E = Cfg::E, T = Cfg::T, COLS = Cfg::COLS;
+  constexpr bool FWD = PHASE == 0 || PHASE == 1 || PHASE == 3;  // forward FFT_y
+  constexpr bool INV = PHASE == 0 || PHASE == 2 || PHASE == 4;  // H, inverse FFT_y
+  constexpr bool ADJW = PHASE == 3 || PHASE == 4;
   // each thread holds P adjacent columns; the LC lane columns of a 16-lane
   // group are the LDS lines it touches together
   constexpr int P = Cfg::P, LC = Cfg::LC;
@@ -466,8 +529,26 @@ k_col(ColArgs a) {
   // H depends on kx^2 and ky^2 only, so H[j][i] = H[(ny-j)%ny][(nx-i)%nx]
   // holds bit for bit; reading the quadrant j<=ny/2, i<=nx/2 keeps the
   // table's footprint at a quarter (L2-resident per XCD strip).
+  // adjoint: the forward k-space plane U (and the weight slice) of the step
+  float2 uv[P][E];
+  float dv[P][E];
+  if constexpr (ADJW) {
+    const float2* uk = a.uk + (size_t)b * a.uk_bstride + x0;
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+#pragma unroll
+      for (int m = 0; m < E; ++m) uv[q][m] = uk[(size_t)(t + m * T) * a.nx + q];
+    if constexpr (PHASE == 4) {
+      const size_t s0 = (size_t)b * a.dn_bstride + x0;
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+          dv[q][m] = a.dn ? load_dn(a.dn, a.dn_kind, s0 + (size_t)(t + m * T) * a.nx + q).x : 0.f;
+    }
+  }
   float2 hv[P][E];
-  if (PHASE != 1 && a.hmode == 0) {
+  if (INV && a.hmode == 0) {
 #pragma unroll
     for (int q = 0; q < P; ++q) {
       const int qx = min(x0 + q, a.nx - x0 - q);
@@ -481,12 +562,30 @@ k_col(ColArgs a) {
 
   float dn0w = 0.f;
   if constexpr (PHASE == 2) {
-    __shared__ float dn0_slot;
-    dn0w = weighted_dn0_group(a.wpart + (size_t)b * a.nx, a.nx, &dn0_slot);
-    if (bx == 0 && threadIdx.x == 0) a.dn0_item[b] = dn0w;
+    __shared__ float dn0_slot[2];
+    dn0w = weighted_dn0_group(a.wpart + (size_t)b * a.nx, a.nx, dn0_slot);
+    if (bx == 0 && threadIdx.x == 0) {
+      a.dn0_item[b] = dn0w;
+      if (a.rec_c) {
+        a.rec_c[b] = dn0w;
+        a.rec_w[b] = dn0_slot[1];
+      }
+    }
+  }
+  if constexpr (ADJW) dn0w = a.cz[b];  // the forward's dn0 of this step
+  float cgrad = 0.f, inv_w = 0.f, n_pts = 0.f;
+  if constexpr (PHASE == 4) {
+    // dL/ddn0 = -sum_x (screen gradient) + dz (n0+dn0) k0^2 / N sum_k Im(conj(X) H U)/beta
+    __shared__ float cg_slot[2];
+    const float2 sums = group_sum_pair(a.s1part + (size_t)b * NY, NY, a.s2part + (size_t)b * a.nx,
+                                       a.nx, cg_slot);
+    n_pts = (float)a.nx * (float)NY;
+    const float n = a.n0 + dn0w;
+    cgrad = -sums.x + a.dz * n * a.k0 * a.k0 / n_pts * sums.y;
+    inv_w = 1.f / a.wz[b];
   }
 
-  if constexpr (PHASE != 2) {
+  if constexpr (FWD) {
 #pragma unroll
     for (int q = 0; q < P; ++q) line_fft<NY, E, false, CL>(v[q], lds[q * LC + c], t, tw);
   }
@@ -508,6 +607,23 @@ k_col(ColArgs a) {
       }
       acc = column_sum2<T>(acc, line, t);
       if (t == 0) a.wpart[(size_t)b * a.nx + x] = acc;
+    } else if constexpr (PHASE == 3) {
+      // dH/ddn0 = H * (-i dz (n0+dn0) k0^2 / beta): collect Im(conj(X) H U)/beta
+      const float an = n_term_f32(f_add(a.n0, dn0w), a.k0);
+      const float kx2 = kcomp2_f32(x, a.nx, a.dx);
+      float acc = 0.f;
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int ky = t + m * T;
+        float beta;
+        const float2 h = prop_f32_beta(kx2, kcomp2_f32(ky, NY, a.dy), an, a.dz, &beta);
+        if (beta > 0.f) {
+          const float2 hu = c_mul(h, uv[q][m]);
+          acc += (v[q][m].x * hu.y - v[q][m].y * hu.x) / beta;
+        }
+      }
+      const float2 tot = column_sum2<T>(make_float2(acc, 0.f), line, t);
+      if (t == 0) a.s2part[(size_t)b * a.nx + x] = tot.x;
     } else {
       float an = 0.f, kx2 = 0.f;
       if (a.hmode != 0) {
@@ -521,6 +637,23 @@ k_col(ColArgs a) {
         const float2 h =
             (a.hmode == 0) ? hv[q][m] : prop_f32(kx2, kcomp2_f32(ky, NY, a.dy), an, a.dz);
         v[q][m] = a.conj_h ? c_mulc(v[q][m], h) : c_mul(v[q][m], h);
+      }
+      if constexpr (PHASE == 4) {
+        // dn0 = sum|U| d / sum|U| depends on U through |U|:
+        // G_U += dL/ddn0 (d - dn0) / W * U/|U| (times N: X is N G_V), and
+        // dL/dd += dL/ddn0 |U| / W
+        float* gd = a.gd_item + (size_t)b * NY * a.nx + x;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const int ky = t + m * T;
+          const float au = hypotf(uv[q][m].x, uv[q][m].y);
+          if (au > 0.f) {
+            const float coef = n_pts * cgrad * (dv[q][m] - dn0w) * inv_w / au;
+            v[q][m].x += coef * uv[q][m].x;
+            v[q][m].y += coef * uv[q][m].y;
+          }
+          gd[(size_t)ky * a.nx] += cgrad * au * inv_w;
+        }
       }
       if (a.sw) {
         // scattering reductions of bpm_3d.py:243-247,335-341 on the k-space
@@ -544,16 +677,26 @@ k_col(ColArgs a) {
       }
     }
   }
+  if constexpr (PHASE == 1) {
+    // weighted forward recorded for the adjoint: U = F u of this step
+    if (a.kout) {
+      float2* ko = a.kout + (size_t)b * a.kout_bstride + x0;
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+#pragma unroll
+        for (int m = 0; m < E; ++m) ko[(size_t)(t + m * T) * a.nx + q] = v[q][m];
+    }
+  }
   if constexpr (SPLIT) {
     // each column stored as soon as its inverse transform is done
 #pragma unroll
     for (int q = 0; q < P; ++q) {
-      if constexpr (PHASE != 1) line_fft<NY, E, true, CL>(v[q], lds[q * LC + c], t, tw);
+      if constexpr (INV) line_fft<NY, E, true, CL>(v[q], lds[q * LC + c], t, tw);
 #pragma unroll
       for (int m = 0; m < E; ++m) p[(size_t)(t + m * T) * a.nx + q] = v[q][m];
     }
   } else {
-    if constexpr (PHASE != 1) {
+    if constexpr (INV) {
 #pragma unroll
       for (int q = 0; q < P; ++q) line_fft<NY, E, true, CL>(v[q], lds[q * LC + c], t, tw);
     }
@@ -973,6 +1116,19 @@ static col_fn pick_col_n(int v) {
   }
 }
 
+// Column phases of the weighted adjoint: one launch shape (V = 1) in the
+// radix schedule of column variant v (its twiddle table is the plan's).
+template <int PHASE>
+static col_fn pick_col_adj(int n, int v) {
+#define COLA_CASE(N)                                                                   \
+  case N:                                                                              \
+    return v >= kColShapes ? launch_col<N, 1, PHASE, col_e_of(N, 1)>                  \
+                           : launch_col<N, 1, PHASE, col_e_of(N, 0)>;
+  switch (n) { BPMH_SIZES(COLA_CASE) }
+#undef COLA_CASE
+  return nullptr;
+}
+
 template <int PHASE>
 static col_fn pick_col(int n, int v) {
 #define COL_CASE(N) \
@@ -1336,6 +1492,153 @@ static int prop_run(bpmh_plan plan, const PropCall& c, const StepScratch& sc, hi
   return BPMH_OK;
 }
 
+// ----------------------------------------------------------------------------
+// memory floor of a step: the two passes' global loads and stores in their
+// launch shapes, without the transforms
+// ----------------------------------------------------------------------------
+// Row pattern: each line of the row pass read and written with streaming
+// accesses and its dn row read (20 B per voxel), one multiply in between.
+template <int NX, int V, int EE>
+__global__ void __launch_bounds__((RowCfg<NX, V, EE>::THREADS))
+k_floor_row(float2* __restrict__ plane, const float* __restrict__ dn, int ny, int z) {
+  using Cfg = RowCfg<NX, V, EE>;
+  constexpr int E = Cfg::E, T = Cfg::T, LINES = Cfg::LINES;
+  const int line = threadIdx.x / T;
+  const int t = threadIdx.x - line * T;
+  const int row = blockIdx.x * LINES + line;
+  const int y = row % ny;
+  float2* p = plane + (size_t)row * NX;
+  const float* d = dn + ((size_t)z * ny + y) * NX;
+  float2 v[E];
+  float dv[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = ld_stream(p + t + m * T);
+#pragma unroll
+  for (int m = 0; m < E; ++m) dv[m] = d[t + m * T];
+#pragma unroll
+  for (int m = 0; m < E; ++m) st_stream(p + t + m * T, c_scale(v[m], 1.f + dv[m]));
+}
+
+// Column pattern: the column pass's tiles (XCD-aware tile order, P columns
+// per thread with 16-byte accesses when P = 2) read and written, 16 B per
+// voxel.
+template <int NY, int V, int EE>
+__global__ void __launch_bounds__((ColCfg<NY, V, EE>::THREADS)) k_floor_col(float2* __restrict__ plane,
+                                                                          int nx) {
+  using Cfg = ColCfg<NY, V, EE>;
+  constexpr int E = Cfg::E, T = Cfg::T, COLS = Cfg::COLS, P = Cfg::P, LC = Cfg::LC;
+  const int ntiles = nx / COLS;
+  const int bx = blockIdx.x;
+  const int tile = (ntiles % 8 == 0) ? (bx & 7) * (ntiles >> 3) + (bx >> 3) : bx;
+  const int c = threadIdx.x % LC;
+  const int t = threadIdx.x / LC;
+  float2* p = plane + (size_t)blockIdx.y * NY * nx + tile * COLS + c * P;
+  constexpr bool WIDE = P == 2 && !Cfg::SPLIT;  // 16-byte accesses, as the pass
+  float2 v[P][E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    float2* r = p + (size_t)(t + m * T) * nx;
+    if constexpr (WIDE) {
+      const float4 w = *reinterpret_cast<const float4*>(r);
+      v[0][m] = make_float2(w.x, w.y);
+      v[1][m] = make_float2(w.z, w.w);
+    } else {
+#pragma unroll
+      for (int q = 0; q < P; ++q) v[q][m] = r[q];
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    float2* r = p + (size_t)(t + m * T) * nx;
+    if constexpr (WIDE) {
+      *reinterpret_cast<float4*>(r) = make_float4(-v[0][m].x, v[0][m].y, -v[1][m].x, v[1][m].y);
+    } else {
+#pragma unroll
+      for (int q = 0; q < P; ++q) r[q] = make_float2(-v[q][m].x, v[q][m].y);
+    }
+  }
+}
+
+template <int N, int V, int E>
+static void launch_floor_row(float2* plane, const float* dn, int batch, int ny, int z,
+                             hipStream_t s) {
+  hipLaunchKernelGGL((k_floor_row<N, V, E>), dim3(batch * ny / RowCfg<N, V, E>::LINES),
+                     dim3(RowCfg<N, V, E>::THREADS), 0, s, plane, dn, ny, z);
+}
+
+template <int N, int V, int E>
+static void launch_floor_col(float2* plane, int batch, int nx, hipStream_t s) {
+  hipLaunchKernelGGL((k_floor_col<N, V, E>), dim3(nx / ColCfg<N, V, E>::COLS, batch),
+                     dim3(ColCfg<N, V, E>::THREADS), 0, s, plane, nx);
+}
+
+typedef void (*floor_row_fn)(float2*, const float*, int, int, int, hipStream_t);
+typedef void (*floor_col_fn)(float2*, int, int, hipStream_t);
+
+template <int N>
+static floor_row_fn pick_floor_row_n(int v) {
+  constexpr int E0 = row_e_of(N, 0), E1 = row_e_of(N, 1);
+  switch (v) {
+    case 0: return launch_floor_row<N, 0, E0>;
+    case 2: return launch_floor_row<N, 2, E0>;
+    case 3: return launch_floor_row<N, 0, E1>;
+    case 4: return launch_floor_row<N, 1, E1>;
+    case 5: return launch_floor_row<N, 2, E1>;
+    default: return launch_floor_row<N, 1, E0>;
+  }
+}
+
+template <int N>
+static floor_col_fn pick_floor_col_n(int v) {
+  constexpr int E0 = col_e_of(N, 0), E1 = col_e_of(N, 1);
+  switch (v) {
+    case 0: return launch_floor_col<N, 0, E0>;
+    case 2: return launch_floor_col<N, 2, E0>;
+    case 3: return launch_floor_col<N, 3, E0>;
+    case 4: return launch_floor_col<N, 4, E0>;
+    case 5: return launch_floor_col<N, 0, E1>;
+    case 6: return launch_floor_col<N, 1, E1>;
+    case 7: return launch_floor_col<N, 2, E1>;
+    case 8: return launch_floor_col<N, 3, E1>;
+    case 9: return launch_floor_col<N, 4, E1>;
+    default: return launch_floor_col<N, 1, E0>;
+  }
+}
+
+extern "C" int bpmh_floor_steps(bpmh_plan plan, int batch, int nz, float* plane_, const float* dn,
+                                long long dn_numel, void* stream) {
+  if (!plan || !plane_ || !dn) return bpmh_set_error(BPMH_ERR_INVALID, "null argument");
+  if (batch < 1 || nz < 1 || batch > 65535 || (long long)batch * plan->ny > (1LL << 30))
+    return bpmh_set_error(BPMH_ERR_INVALID, "bad batch or depth");
+  const int nx = plan->nx, ny = plan->ny;
+  if (dn_numel < (long long)nz * nx * ny)
+    return bpmh_set_error(BPMH_ERR_INVALID, "dn holds fewer than nz planes");
+  float2* plane = (float2*)plane_;
+  const int cv = col_variant_for(plan->col_v, plane);
+  floor_row_fn fr = nullptr;
+  floor_col_fn fc = nullptr;
+#define FLOOR_CASE(N)                          \
+  case N:                                      \
+    fr = pick_floor_row_n<N>(plan->row_v);     \
+    break;
+  switch (nx) { BPMH_SIZES(FLOOR_CASE) }
+#undef FLOOR_CASE
+#define FLOOR_CASE(N)                  \
+  case N:                              \
+    fc = pick_floor_col_n<N>(cv);      \
+    break;
+  switch (ny) { BPMH_SIZES(FLOOR_CASE) }
+#undef FLOOR_CASE
+  if (!fr || !fc) return bpmh_set_error(BPMH_ERR_UNSUPPORTED, "grid size");
+  hipStream_t s = (hipStream_t)stream;
+  for (int z = 1; z < nz; ++z) {
+    fc(plane, batch, nx, s);
+    fr(plane, dn, batch, ny, z, s);
+  }
+  HIP_TRY(hipGetLastError());
+  return BPMH_OK;
+}
+
 extern "C" int bpmh_propagate(bpmh_plan plan, int batch, int nz, float* plane, const void* dn,
                               int dn_kind, long long dn_batch_stride, long long dn_numel,
                               int mean_mode, const float* H, const float* n_eff, const float* dn0,
@@ -1689,6 +1992,235 @@ extern "C" int bpmh_adjoint_segment(bpmh_plan plan, int batch, int z_lo, int z_h
   ra.inverse = 1;
   ra.fwd_items = 0;
   row(ra, batch, s);
+  HIP_TRY(hipGetLastError());
+  return BPMH_OK;
+}
+
+// ----------------------------------------------------------------------------
+// weighted-mean segment of the differentiable form
+// ----------------------------------------------------------------------------
+// out[r] = sum of row r (nx floats) of g; one wavefront per row, fixed order
+__global__ void __launch_bounds__(256) k_row_sums(const float* __restrict__ g,
+                                                  float* __restrict__ out, int rows, int nx) {
+  const int r = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= rows) return;  // whole wavefront
+  const float* p = g + (size_t)r * nx;
+  float sum = 0.f;
+  for (int i = lane; i < nx; i += 64) sum += p[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+  if (lane == 0) out[r] = sum;
+}
+
+__global__ void k_fill_zero(float* __restrict__ p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.f;
+}
+
+// Workspace of bpmh_weighted_segment, in floats (each part 256-byte aligned).
+struct WsegLayout {
+  size_t wpart, dn0, fplane, useg, kseg, recc, recw, s1, s2, gitem, gd, total;
+};
+
+static WsegLayout wseg_layout(int nx, int ny, int batch, int nseg, bool adjoint) {
+  WsegLayout L;
+  memset(&L, 0, sizeof L);
+  size_t o = 0;
+  auto take = [&o](size_t n) {
+    const size_t r = o;
+    o += (n + 63) & ~(size_t)63;
+    return r;
+  };
+  const size_t n2 = (size_t)nx * ny, B = (size_t)batch, S = (size_t)nseg;
+  L.wpart = take(2 * B * nx);
+  L.dn0 = take(B);
+  if (adjoint) {
+    L.fplane = take(2 * B * n2);
+    L.useg = take(2 * B * S * n2);
+    L.kseg = take(2 * B * (S - 1) * n2);
+    L.recc = take((S - 1) * B);
+    L.recw = take((S - 1) * B);
+    L.s1 = take(B * ny);
+    L.s2 = take(B * nx);
+    L.gitem = take(B * n2);
+    L.gd = take(B * n2);
+  }
+  L.total = o;
+  return L;
+}
+
+extern "C" int bpmh_weighted_segment_work(bpmh_plan plan, int batch, int z_lo, int z_hi,
+                                          int adjoint, long long* numel) {
+  if (!plan || !numel) return bpmh_set_error(BPMH_ERR_INVALID, "null argument");
+  if (batch < 1 || z_lo < 0 || z_hi < z_lo) return bpmh_set_error(BPMH_ERR_INVALID, "bad range");
+  *numel = (long long)wseg_layout(plan->nx, plan->ny, batch, z_hi - z_lo + 1, adjoint != 0).total;
+  return BPMH_OK;
+}
+
+extern "C" int bpmh_weighted_segment(bpmh_plan plan, int batch, int z_lo, int z_hi, float* plane_,
+                                     const float* dn, long long dn_numel, float n0, float k0,
+                                     float dx, float dy, float dz, float unit_k, int absorb,
+                                     float* g_, float* grad_dn, float* work, long long work_numel,
+                                     void* stream) {
+  if (!plan || !plane_ || !dn || !work) return bpmh_set_error(BPMH_ERR_INVALID, "null argument");
+  const bool adjoint = g_ != nullptr;
+  if (adjoint && !grad_dn) return bpmh_set_error(BPMH_ERR_INVALID, "the adjoint needs grad_dn");
+  if (batch < 1 || z_lo < 0 || z_hi < z_lo) return bpmh_set_error(BPMH_ERR_INVALID, "bad range");
+  if (batch > 65535 || (long long)batch * plan->ny > (1LL << 30))
+    return bpmh_set_error(BPMH_ERR_INVALID, "batch too large");
+  if (absorb < 0) return bpmh_set_error(BPMH_ERR_INVALID, "absorbing width must be >= 0");
+  const int nx = plan->nx, ny = plan->ny;
+  const int nseg = z_hi - z_lo + 1;
+  const long long n2 = (long long)nx * ny;
+  char msg[256];
+  if (dn_numel < (long long)(z_hi + 1) * n2) {
+    snprintf(msg, sizeof msg, "dn holds %lld elements; planes up to %d need %lld", dn_numel, z_hi,
+             (long long)(z_hi + 1) * n2);
+    return bpmh_set_error(BPMH_ERR_INVALID, msg);
+  }
+  const WsegLayout L = wseg_layout(nx, ny, batch, nseg, adjoint);
+  if (work_numel < (long long)L.total) {
+    snprintf(msg, sizeof msg, "work holds %lld floats; this segment needs %lld", work_numel,
+             (long long)L.total);
+    return bpmh_set_error(BPMH_ERR_INVALID, msg);
+  }
+  if (nseg == 1) return BPMH_OK;
+  const int urc = plan_upload(plan);
+  if (urc) return urc;
+  hipStream_t s = (hipStream_t)stream;
+  const row_fn row = pick_row(nx, plan->row_v);
+  float2* plane = (float2*)plane_;
+  float2* fp = adjoint ? (float2*)(work + L.fplane) : plane;
+  const int cv = col_variant_for(plan->col_v, fp);
+  const col_fn colA = pick_col<1>(ny, cv);
+  const col_fn colB = pick_col<2>(ny, cv);
+  const size_t B = (size_t)batch, S = (size_t)nseg;
+  float2* useg = (float2*)(work + L.useg);
+  float2* kseg = (float2*)(work + L.kseg);
+  if (adjoint)
+    HIP_TRY(hipMemcpyAsync(fp, plane, B * n2 * sizeof(float2), hipMemcpyDeviceToDevice, s));
+
+  // forward z_lo -> z_hi (recorded when the adjoint follows); the weights
+  // read slice 0 of the whole volume, the screens dn[z_lo + z]
+  RowArgs ra;
+  memset(&ra, 0, sizeof ra);
+  ra.plane = fp;
+  ra.tw = plan->tw_x[plan->row_v / kRowShapes];
+  ra.dn = dn + (size_t)z_lo * n2;
+  ra.dn_kind = BPMH_DN_F32;
+  ra.ny = ny;
+  ra.nz = nseg;
+  ra.unit_k = unit_k;
+  ra.inv_n = 1.0f / ((float)nx * (float)ny);
+  ra.absorb = absorb;
+  ra.dn0_mode = BPMH_MEAN_WEIGHTED;
+  ra.dn0_item = work + L.dn0;
+  ra.out_field = adjoint ? useg : nullptr;
+
+  ColArgs ca;
+  memset(&ca, 0, sizeof ca);
+  ca.plane = fp;
+  ca.tw = plan->tw_y[cv / kColShapes];
+  ca.nx = nx;
+  ca.nz = nseg;
+  ca.hmode = BPMH_MEAN_WEIGHTED;
+  ca.n0 = n0;
+  ca.k0 = k0;
+  ca.dx = dx;
+  ca.dy = dy;
+  ca.dz = dz;
+  ca.dn = dn;
+  ca.dn_kind = BPMH_DN_F32;
+  ca.wpart = (float2*)(work + L.wpart);
+  ca.dn0_item = work + L.dn0;
+
+  ra.z = 0;
+  ra.fwd_items = batch;
+  row(ra, batch, s);
+  ra.inverse = 1;
+  ra.screen = 1;
+  for (int z = 1; z < nseg; ++z) {
+    ca.z = z;
+    if (adjoint) {
+      ca.kout = kseg + (size_t)(z - 1) * n2;
+      ca.kout_bstride = (long long)(S - 1) * n2;
+      ca.rec_c = work + L.recc + (size_t)(z - 1) * B;
+      ca.rec_w = work + L.recw + (size_t)(z - 1) * B;
+    }
+    colA(ca, batch, s);
+    colB(ca, batch, s);
+    ra.z = z;
+    ra.fwd_items = z < nseg - 1 ? batch : 0;
+    row(ra, batch, s);
+  }
+  if (!adjoint) {
+    HIP_TRY(hipGetLastError());
+    return BPMH_OK;
+  }
+
+  // adjoint z_hi -> z_lo
+  float2* g = (float2*)g_;
+  float* gitem = work + L.gitem;
+  float* gd = work + L.gd;
+  const size_t nn = (size_t)n2;
+  const unsigned fill_blocks = (unsigned)((B * nn + 255) / 256);
+  hipLaunchKernelGGL(k_fill_zero, dim3(fill_blocks), dim3(256), 0, s, gd, B * nn);
+  const col_fn col3 = pick_col_adj<3>(ny, cv);
+  const col_fn col4 = pick_col_adj<4>(ny, cv);
+  RowArgs rb = ra;
+  rb.plane = g;
+  rb.out_field = nullptr;
+  rb.adjoint = 1;
+  rb.useg = useg;
+  rb.useg_bstride = (long long)S * n2;
+  rb.gitem = gitem;
+  rb.screen = 1;
+  rb.fwd_items = batch;
+  ColArgs cb;
+  memset(&cb, 0, sizeof cb);
+  cb.plane = g;
+  cb.tw = ca.tw;
+  cb.nx = nx;
+  cb.nz = nseg;
+  cb.hmode = BPMH_MEAN_WEIGHTED;
+  cb.conj_h = 1;
+  cb.n0 = n0;
+  cb.k0 = k0;
+  cb.dx = dx;
+  cb.dy = dy;
+  cb.dz = dz;
+  cb.dn = dn;
+  cb.dn_kind = BPMH_DN_F32;
+  cb.uk_bstride = (long long)(S - 1) * n2;
+  cb.s2part = work + L.s2;
+  cb.s1part = work + L.s1;
+  cb.gd_item = gd;
+  const unsigned red_blocks = (unsigned)((nn + 255) / 256);
+  for (int z = nseg - 1; z >= 1; --z) {
+    rb.z = z;
+    rb.useg_z = z;
+    rb.inverse = z != nseg - 1;
+    rb.dn0_item = work + L.recc + (size_t)(z - 1) * B;
+    row(rb, batch, s);
+    hipLaunchKernelGGL(k_grad_reduce, dim3(red_blocks), dim3(256), 0, s, gitem,
+                       grad_dn + (size_t)(z_lo + z) * nn, batch, nn);
+    hipLaunchKernelGGL(k_row_sums, dim3((unsigned)((B * ny + 3) / 4)), dim3(256), 0, s, gitem,
+                       work + L.s1, (int)(B * ny), nx);
+    cb.z = z;
+    cb.uk = kseg + (size_t)(z - 1) * n2;
+    cb.cz = work + L.recc + (size_t)(z - 1) * B;
+    cb.wz = work + L.recw + (size_t)(z - 1) * B;
+    col3(cb, batch, s);
+    col4(cb, batch, s);
+  }
+  rb.adjoint = 0;
+  rb.screen = 0;
+  rb.inverse = 1;
+  rb.fwd_items = 0;
+  row(rb, batch, s);
+  // the weight slice is slice 0 of the volume
+  hipLaunchKernelGGL(k_grad_reduce, dim3(red_blocks), dim3(256), 0, s, gd, grad_dn, batch, nn);
   HIP_TRY(hipGetLastError());
   return BPMH_OK;
 }

@@ -114,12 +114,22 @@ struct DwOut {
   float I;
 };
 
-BPMH_DEV DwOut dw_fields(const DwArgs& a, float phi, float2 I0, float2 I1, float2 I2) {
+// cos 2phi, sin 2phi, cos phi, sin phi of a lateral point (float64
+// evaluation, rounded), computed once per point and shared by its planes.
+struct DwTrig {
+  float c2, s2, cp, sp;
+};
+
+BPMH_DEV DwTrig dw_trig(float phi) {
   double s2d, c2d, s1d, c1d;
   const float phi2 = f_mul(2.f, phi);
   sincos((double)phi2, &s2d, &c2d);
   sincos((double)phi, &s1d, &c1d);
-  const float c2 = (float)c2d, s2 = (float)s2d, cp = (float)c1d, sp = (float)s1d;
+  return DwTrig{(float)c2d, (float)s2d, (float)c1d, (float)s1d};
+}
+
+BPMH_DEV DwOut dw_fields(const DwArgs& a, const DwTrig& tr, float2 I0, float2 I1, float2 I2) {
+  const float c2 = tr.c2, s2 = tr.s2, cp = tr.cp, sp = tr.sp;
   const float ey0s2 = f_mul(a.ey0, s2), ex0s2 = f_mul(a.ex0, s2);
   DwOut o;
   o.ex = make_float2(f_add(f_mul(a.ex0, f_add(I0.x, f_mul(I2.x, c2))), f_mul(ey0s2, I2.x)),
@@ -135,8 +145,9 @@ BPMH_DEV DwOut dw_fields(const DwArgs& a, float phi, float2 I0, float2 I1, float
   return o;
 }
 
-BPMH_DEV void combine(const DwArgs& a, float phi, float2 I0, float2 I1, float2 I2, size_t o) {
-  const DwOut v = dw_fields(a, phi, I0, I1, I2);
+BPMH_DEV void combine(const DwArgs& a, const DwTrig& tr, float2 I0, float2 I1, float2 I2,
+                      size_t o) {
+  const DwOut v = dw_fields(a, tr, I0, I1, I2);
   if (a.ex) a.ex[o] = v.ex;
   if (a.ey) a.ey[o] = v.ey;
   if (a.ez) a.ez[o] = v.ez;
@@ -334,6 +345,8 @@ BPMH_DEV void dw_mfma_tiles(const DwArgs& a, int ncols, int zp, int tp, int cp, 
                                              (double)axis_coord(j, a.nx, a.x1, a.x2))
                               : 0.f;
     const size_t o1 = (size_t)j * a.nx + i, o2 = (size_t)i * a.nx + j;
+    const DwTrig tr1 = dw_trig(phi1);
+    const DwTrig tr2 = second ? dw_trig(phi2) : tr1;
 #pragma unroll
     for (int r = 0; r < ZT; ++r)
 #pragma unroll
@@ -343,24 +356,40 @@ BPMH_DEV void dw_mfma_tiles(const DwArgs& a, int ncols, int zp, int tp, int cp, 
         const float2 I0 = make_float2(acc[r][t][0][q], acc[r][t][1][q]);
         const float2 I1 = make_float2(acc[r][t][2][q], acc[r][t][3][q]);
         const float2 I2 = make_float2(acc[r][t][4][q], acc[r][t][5][q]);
-        combine(a, phi1, I0, I1, I2, (size_t)z * out_plane + o1);
-        if (second) combine(a, phi2, I0, I1, I2, (size_t)z * out_plane + o2);
+        combine(a, tr1, I0, I1, I2, (size_t)z * out_plane + o1);
+        if (second) combine(a, tr2, I0, I1, I2, (size_t)z * out_plane + o2);
       }
   }
 }
 
-// Wavefront w of work-group (x, y) owns the 16-plane tile 4y + w and the
-// table columns 32x .. 32x+31.  (Two plane tiles per wavefront reuse each
-// loaded column operand twice but need 220 registers, two wavefronts per
-// SIMD, and leave a tail of unequal wavefronts: 0.364 vs 0.341 ms per 256^3
-// call.)
-// The launch covers the work-group rows from zrow0 on (grid y rows).
+// Wavefronts per product work-group for nzt plane tiles: the W in 4, 3, 2
+// that leaves the fewest idle wavefronts (larger W on ties), or nzt itself
+// when it is below 4.
+static int dw_waves_per_group(int nzt) {
+  if (nzt <= 4) return nzt < 1 ? 1 : nzt;
+  int best = 4, idle = (4 - nzt % 4) % 4;
+  for (int w = 3; w >= 2; --w) {
+    const int i = (w - nzt % w) % w;
+    if (i < idle) best = w, idle = i;
+  }
+  return best;
+}
+
+// Wavefront w of a work-group of W wavefronts (W = blockDim.x / 64) at (x, y)
+// owns the 16-plane tile W*y + w and the table columns 32x .. 32x+31.  W is
+// chosen by the host so that W divides the number of plane tiles when it
+// can: every wavefront then has work and, with three wavefronts per SIMD
+// (154 registers), all work-groups of a 256^3 call are resident at once
+// (four-wavefront groups over nine plane tiles left 21 groups for a second
+// round).  (Two plane tiles per wavefront reuse each loaded column operand
+// twice but need 220 registers, two wavefronts per SIMD, and leave a tail of
+// unequal wavefronts: 0.364 vs 0.341 ms per 256^3 call.)
 template <bool PAIRS>
 __global__ void __launch_bounds__(256, 2) k_dw_mfma(DwArgs a, int ncols, int nzt, int tp, int cp,
                                                 const float2* __restrict__ P,
-                                                const float* __restrict__ A, int zrow0) {
+                                                const float* __restrict__ A) {
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int zt0 = (blockIdx.y + zrow0) * 4 + w;
+  const int zt0 = blockIdx.y * (int)(blockDim.x >> 6) + w;
   if (zt0 >= nzt) return;  // whole wavefront; the kernel has no barriers
   const int c0 = blockIdx.x * (16 * DM_CT);
   dw_mfma_tiles<1, PAIRS>(a, ncols, nzt * 16, tp, cp, zt0, c0, l, P, A);
@@ -384,7 +413,7 @@ __global__ void __launch_bounds__(64) k_debye_points(DwArgs a, int n) {
       I2.x += sm.a2 * cs; I2.y += sm.a2 * sn;
     }
   }
-  combine(a, phi, I0, I1, I2, (size_t)p);
+  combine(a, dw_trig(phi), I0, I1, I2, (size_t)p);
 }
 
 // Full-grid planes of one launch of the spread: blockIdx.z < n_hi gives
@@ -623,9 +652,12 @@ static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int ncols = nx * ny;
   // samples index the y extent of two grids (at most 65535); the product
-  // grid's y extent is about nz / 128
+  // grid's y extent is the plane tiles over the wavefronts per work-group
   const long long total_ll = ((long long)int_steps + 1) * (n_alphas / 2);
-  if (total_ll > 65000 || nz > (1 << 23) || (long long)ncols * (total_ll + 64) > (1LL << 31) ||
+  const int nzt_chk = nz > (1 << 23) ? 0 : (nz + 15) / 16;
+  if (total_ll > 65000 || nz > (1 << 23) ||
+      (nzt_chk + dw_waves_per_group(nzt_chk) - 1) / dw_waves_per_group(nzt_chk) > 65535 ||
+      (long long)ncols * (total_ll + 64) > (1LL << 31) ||
       (long long)(nz + 16) * (total_ll + 64) > (1LL << 31))
     return bpmh_set_error(BPMH_ERR_INVALID, "grid x samples too large");
   const int total = (int)total_ll;
@@ -675,14 +707,15 @@ static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st
     hipLaunchKernelGGL(k_dw_phases, dim3((zp + 63) / 64, tp), dim3(64), 0, st, a, total, tp, zp,
                        (const float*)co, P);
   });
-  const dim3 mgrid(cp / (16 * DM_CT), (nzt + 3) / 4);
+  const int wpg = dw_waves_per_group(nzt);
+  const dim3 mgrid(cp / (16 * DM_CT), (nzt + wpg - 1) / wpg);
   if (!rc) rc = dw_timed(3, st, [&] {
     if (pairs)
-      hipLaunchKernelGGL(k_dw_mfma<true>, mgrid, dim3(256), 0, st, a, ntab, nzt, tp, cp,
-                         (const float2*)P, (const float*)A, 0);
+      hipLaunchKernelGGL(k_dw_mfma<true>, mgrid, dim3(64 * wpg), 0, st, a, ntab, nzt, tp, cp,
+                         (const float2*)P, (const float*)A);
     else
-      hipLaunchKernelGGL(k_dw_mfma<false>, mgrid, dim3(256), 0, st, a, ncols, nzt, tp, cp,
-                         (const float2*)P, (const float*)A, 0);
+      hipLaunchKernelGGL(k_dw_mfma<false>, mgrid, dim3(64 * wpg), 0, st, a, ncols, nzt, tp, cp,
+                         (const float2*)P, (const float*)A);
   });
   if (!rc && full)
     rc = dw_timed(4, st, [&] {

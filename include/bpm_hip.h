@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define BPMH_VERSION 200
+#define BPMH_VERSION 201
 
 #define BPMH_OK 0
 #define BPMH_ERR_INVALID -1      /* bad argument  (Python: ValueError) */
@@ -98,6 +98,17 @@ int bpmh_plan_set_graph(bpmh_plan plan, int enable);
  * get_variant also reports the lines per row work-group, columns per column
  * tile and elements per thread of each pass.  No reference counterpart
  * (gputools picks its own FFT plans, bpm/bpm/bpm_3d.py:188). */
+/* Memory floor of nz-1 z-steps on this plan (measurement only): per step
+ * the column pass's and then the row pass's global loads and stores in the
+ * plan's current launch shapes -- the plane read and written by both, one
+ * float32 dn slice (nz, ny, nx; dn_numel elements) read by the row pattern,
+ * 36 B per voxel-step -- with no transforms; the plane's values are
+ * scrambled.  A propagation's step time over this one's is the cost of the
+ * transforms on top of the step's data movement.  No reference
+ * counterpart. */
+int bpmh_floor_steps(bpmh_plan plan, int batch, int nz, float* plane, const float* dn,
+                     long long dn_numel, void* stream);
+
 int bpmh_plan_set_variant(bpmh_plan plan, int row_variant, int col_variant);
 int bpmh_plan_get_variant(bpmh_plan plan, int* row_variant, int* col_variant,
                           int* lines_per_group, int* cols_per_group, int* row_elems,
@@ -224,6 +235,33 @@ int bpmh_adjoint_segment(bpmh_plan plan, int batch, int z_lo, int z_hi, float* g
                          const float* u_seg, long long u_seg_numel, const void* dn, int dn_kind,
                          long long dn_numel, const float* H, float unit_k, int absorb,
                          float* grad_dn, float* gitem, void* stream);
+
+/* The differentiable form under bpm_3d's default mean_mode "weighted"
+ * (bpm/bpm/bpm_3d.py:149-159,302-311: dn0 = sum|U| d / sum|U| over the
+ * k-space plane U = F u of each step, with d the volume's slice 0 read at
+ * the same flat index; H(n0 + dn0) and the screen exp(-i k0 dz (dn - dn0))),
+ * one segment z_lo..z_hi of a float32 dn volume (nz, ny, nx) shared by the
+ * batch (dn_numel elements, at least (z_hi+1)*ny*nx); n0, k0, dx, dy, dz,
+ * unit_k, absorb as for bpmh_propagate.
+ * g == NULL: forward -- plane (batch, ny, nx) c64 is advanced in place from
+ *   plane z_lo to plane z_hi, bit-identical to those steps of bpmh_propagate
+ *   with mean_mode WEIGHTED over the whole volume.
+ * g != NULL: adjoint -- plane holds the forward plane z_lo and is only read;
+ *   the segment is recomputed into work (keeping each step's U, dn0 and
+ *   sum|U|), then g (batch, ny, nx) c64 is carried from G_{z_hi} = 2 dL/du*
+ *   of plane z_hi to G_{z_lo}, and grad_dn (nz, ny, nx) f32 receives += dL/ddn
+ *   for planes z_lo+1..z_hi (through the screens) and for slice 0 (through
+ *   the mean's weights).
+ * work: device floats, at least what bpmh_weighted_segment_work reports
+ * (forward: 2*batch*nx + batch; adjoint: about batch*ny*nx*(4*nseg + 4)).
+ * Nothing is allocated; calls sharing a work buffer must not overlap.
+ * No reference counterpart (the reference has no gradient). */
+int bpmh_weighted_segment(bpmh_plan plan, int batch, int z_lo, int z_hi, float* plane,
+                          const float* dn, long long dn_numel, float n0, float k0, float dx,
+                          float dy, float dz, float unit_k, int absorb, float* g, float* grad_dn,
+                          float* work, long long work_numel, void* stream);
+int bpmh_weighted_segment_work(bpmh_plan plan, int batch, int z_lo, int z_hi, int adjoint,
+                               long long* numel);
 
 /* Debye-Wolf kernel variants */
 #define BPMH_DW_INDEX 0  /* debye_wolf, bpm/psf/kernels/psf_debye.cl:24-113 */

@@ -34,7 +34,7 @@ def test_every_declared_symbol_is_exported_and_bound(lib):
     for n in names:
         assert hasattr(lib, n), n
         assert n in _lib.SIGNATURES, "no ctypes signature for " + n
-    assert lib.bpmh_version() == 200
+    assert lib.bpmh_version() == 201
 
 
 def test_plan_rejects_unsupported_sizes_without_gpu(lib):
@@ -177,6 +177,41 @@ def test_ragged_batch_arguments_checked_before_launch(lib):
             (lambda: call([2, 3], [20, 10], 20 * n, 29 * n), b"group 1 reads"),
             (lambda: call([2, 0], [20, 10], 20 * n, 40 * n), b">= 1 item"),
             (lambda: call([1] * 17, [5] * 17, 5 * n, 17 * 5 * n), b"1 to 16 groups"),
+        ]
+        for fn, msg in cases:
+            rc = fn()
+            assert rc == _lib.ERR_INVALID and msg in lib.bpmh_last_error(), (rc, msg)
+    finally:
+        assert lib.bpmh_plan_destroy(h) == 0
+
+
+def test_weighted_segment_and_floor_checked_before_launch(lib):
+    """bpmh_weighted_segment refuses a dn volume short of plane z_hi, a work
+    buffer below what bpmh_weighted_segment_work reports, and an adjoint
+    call without grad_dn; bpmh_floor_steps refuses a dn of fewer than nz
+    planes -- all before touching a device."""
+    h = ctypes.c_void_p()
+    assert lib.bpmh_plan_create(64, 32, ctypes.byref(h)) == 0
+    try:
+        n = 64 * 32
+        fake = ctypes.c_void_p(4096)
+        need = ctypes.c_longlong()
+        assert lib.bpmh_weighted_segment_work(h, 2, 0, 8, 0, ctypes.byref(need)) == 0
+        assert need.value == 256 + 64  # 2*batch*nx partials + batch dn0, 256-byte aligned parts
+        assert lib.bpmh_weighted_segment_work(h, 2, 0, 8, 1, ctypes.byref(need)) == 0
+        adj = need.value
+        # recorded planes (9), k-space planes (8), the forward copy (c64) and two f32 planes
+        assert adj >= 2 * 2 * n * (9 + 8 + 1) + 2 * 2 * n
+
+        def seg(z_hi, dn_numel, g, grad, work_numel):
+            return lib.bpmh_weighted_segment(h, 2, 0, z_hi, fake, fake, dn_numel, 1.33, 12.5, .1, .1,
+                                             .1, 1.25, 0, g, grad, fake, work_numel, None)
+
+        cases = [
+            (lambda: seg(8, 8 * n, None, None, 10 ** 9), b"dn holds"),
+            (lambda: seg(8, 9 * n, fake, fake, adj - 1), b"work holds"),
+            (lambda: seg(8, 9 * n, fake, None, adj), b"needs grad_dn"),
+            (lambda: lib.bpmh_floor_steps(h, 1, 4, fake, fake, 3 * n, None), b"fewer than nz"),
         ]
         for fn, msg in cases:
             rc = fn()
