@@ -44,6 +44,7 @@ struct DwArgs {
   float2 *ex, *ey, *ez;
   float* I;
   const float *px, *py, *pz;  // points mode
+  float2* itab;               // integral table (3, nz, ncols): the product's output when set
 };
 
 // coordinate along one axis: a1 + i*(a2-a1)/(n-1.f) (psf_debye.cl:46-48)
@@ -268,7 +269,7 @@ typedef float dm_v4f __attribute__((ext_vector_type(4)));
 
 // ZT 16-plane tiles (consecutive, from zt0) x DM_CT 16-column tiles per
 // wavefront; each loaded operand feeds ZT (left) or DM_CT (right) MFMAs.
-template <int ZT, bool PAIRS>
+template <int ZT, bool PAIRS, bool ITAB>
 BPMH_DEV void dw_mfma_tiles(const DwArgs& a, int ncols, int zp, int tp, int cp, int zt0, int c0, int l,
                             const float2* __restrict__ P, const float* __restrict__ A) {
   const int lr = l & 15, lk = l >> 4;
@@ -328,6 +329,27 @@ BPMH_DEV void dw_mfma_tiles(const DwArgs& a, int ncols, int zp, int tp, int cp, 
           }
   }
 
+  if constexpr (ITAB) {
+    // the three integrals of each (plane, table column), 24 contiguous bytes,
+    // columns consecutive across the lanes of a tile; the fields are formed
+    // where they are spread
+#pragma unroll
+    for (int t = 0; t < DM_CT; ++t) {
+      const int c = c0 + 16 * t + lr;
+      if (c >= ncols) continue;
+#pragma unroll
+      for (int r = 0; r < ZT; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int z = (zt0 + r) * 16 + 4 * lk + q;
+          if (z >= a.nz) continue;
+          float2* o = a.itab + ((size_t)z * ncols + c) * 3;
+#pragma unroll
+          for (int n = 0; n < 3; ++n) o[n] = make_float2(acc[r][t][2 * n][q], acc[r][t][2 * n + 1][q]);
+        }
+    }
+    return;
+  }
   // table column c is lateral point c, or (PAIRS) the points (i, j) and
   // (j, i) of pair c, each combined with its own azimuth
   const size_t out_plane = (size_t)a.nx * a.ny;
@@ -384,7 +406,7 @@ static int dw_waves_per_group(int nzt) {
 // round).  (Two plane tiles per wavefront reuse each loaded column operand
 // twice but need 220 registers, two wavefronts per SIMD, and leave a tail of
 // unequal wavefronts: 0.364 vs 0.341 ms per 256^3 call.)
-template <bool PAIRS>
+template <bool PAIRS, bool ITAB>
 __global__ void __launch_bounds__(256, 2) k_dw_mfma(DwArgs a, int ncols, int nzt, int tp, int cp,
                                                 const float2* __restrict__ P,
                                                 const float* __restrict__ A) {
@@ -392,7 +414,20 @@ __global__ void __launch_bounds__(256, 2) k_dw_mfma(DwArgs a, int ncols, int nzt
   const int zt0 = blockIdx.y * (int)(blockDim.x >> 6) + w;
   if (zt0 >= nzt) return;  // whole wavefront; the kernel has no barriers
   const int c0 = blockIdx.x * (16 * DM_CT);
-  dw_mfma_tiles<1, PAIRS>(a, ncols, nzt * 16, tp, cp, zt0, c0, l, P, A);
+  dw_mfma_tiles<1, PAIRS, ITAB>(a, ncols, nzt * 16, tp, cp, zt0, c0, l, P, A);
+}
+
+// Azimuth terms of every lateral octant point (row-major, nx * ny):
+// dw_trig of atan2(y_j, x_i), the same values the product's own epilogue
+// uses for the point.
+__global__ void k_dw_trig(DwArgs a, float4* __restrict__ trig) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.nx * a.ny) return;
+  const int i = k % a.nx, j = k / a.nx;
+  const float phi = (float)atan2((double)axis_coord(j, a.ny, a.y1, a.y2),
+                                 (double)axis_coord(i, a.nx, a.x1, a.x2));
+  const DwTrig tr = dw_trig(phi);
+  trig[k] = make_float4(tr.c2, tr.s2, tr.cp, tr.sp);
 }
 
 __global__ void __launch_bounds__(64) k_debye_points(DwArgs a, int n) {
@@ -476,6 +511,83 @@ __global__ void __launch_bounds__(128) k_debye_mirror2(int nx0, int ny0, int nz0
     float2 a = src[q][row + ox0], b = src[q][row + ox1];
     if (conj) { a.y = -a.y; b.y = -b.y; }
     __builtin_nontemporal_store(dw_f4v{a.x, a.y, b.x, b.y}, (dw_f4v*)(dst[q] + sf));
+  }
+}
+
+// The spread from the pair-indexed integral table (identical octant axes).
+// A work-group takes `rows` octant rows oy0.. (all ox) of octant plane oz
+// for one full-grid plane fz (oz = |fz - nz0/2|): it first gathers their
+// integrals into LDS -- pair (min, max) of point (ox, oy), rows i <= j read
+// along ox and the transposed points along oy, so both are runs of
+// consecutive pairs -- then writes the full-grid rows fy = ny0/2 +- oy, two
+// x-neighbours per lane with 16-byte streaming stores.  Each voxel gets the
+// fields of its octant point (ox, oy, oz) formed with that point's azimuth
+// terms -- the values the octant would hold -- conjugated for fz <= nz0/2.
+__global__ void __launch_bounds__(256) k_debye_spread_pairs(DwArgs a, int nx0, int ny0, int nz0,
+                                                            int rows,
+                                                            const float2* __restrict__ itab,
+                                                            const float4* __restrict__ trig,
+                                                            float* u, float2* ex, float2* ey,
+                                                            float2* ez, PlaneMap pm) {
+  extern __shared__ float2 S[];  // [rows][nxo][3]
+  const int nxo = nx0 / 2 + 1;
+  const int npairs = nxo * (nxo + 1) / 2;
+  const int oy0 = blockIdx.x * rows;
+  const int fz = plane_of(pm);
+  const int oz = abs(fz - nz0 / 2);
+  const float2* it = itab + (size_t)oz * npairs * 3;
+  const int nr = min(rows, nxo - oy0);
+  // points with ox <= oy: pair tri(oy) + ox, consecutive along ox
+  for (int e = threadIdx.x; e < nr * nxo; e += blockDim.x) {
+    const int r = e / nxo, ox = e - r * nxo, oy = oy0 + r;
+    if (ox > oy) continue;
+    const float2* q = it + ((size_t)oy * (oy + 1) / 2 + ox) * 3;
+    float2* d = S + ((size_t)r * nxo + ox) * 3;
+    d[0] = q[0];
+    d[1] = q[1];
+    d[2] = q[2];
+  }
+  // points with ox > oy: pair tri(ox) + oy, consecutive along oy
+  for (int e = threadIdx.x; e < nr * nxo; e += blockDim.x) {
+    const int ox = e / nr, r = e - ox * nr, oy = oy0 + r;
+    if (ox <= oy) continue;
+    const float2* q = it + ((size_t)ox * (ox + 1) / 2 + oy) * 3;
+    float2* d = S + ((size_t)r * nxo + ox) * 3;
+    d[0] = q[0];
+    d[1] = q[1];
+    d[2] = q[2];
+  }
+  __syncthreads();
+  const bool conj = fz <= nz0 / 2;
+  const int hx = nx0 / 2, hy = ny0 / 2;
+  const int lanes_per_row = nx0 / 2;  // two x-neighbours per lane
+  // full-grid rows of this block: fy = hy + oy (oy < hy) and hy - oy (oy >= 1)
+  const int nrows = 2 * nr;
+  for (int e = threadIdx.x; e < nrows * lanes_per_row; e += blockDim.x) {
+    const int k = e / lanes_per_row;
+    const int fx = 2 * (e - k * lanes_per_row);
+    const int r = k >> 1, oy = oy0 + r;
+    const int fy = (k & 1) ? hy - oy : hy + oy;
+    if ((k & 1) ? (oy < 1) : (oy >= hy)) continue;
+    DwOut o[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int ox = abs(fx + h - hx);
+      const float4 tv = trig[(size_t)oy * nxo + ox];
+      const DwTrig tr{tv.x, tv.y, tv.z, tv.w};
+      const float2* v = S + ((size_t)r * nxo + ox) * 3;
+      o[h] = dw_fields(a, tr, v[0], v[1], v[2]);
+      if (conj) {
+        o[h].ex.y = -o[h].ex.y;
+        o[h].ey.y = -o[h].ey.y;
+        o[h].ez.y = -o[h].ez.y;
+      }
+    }
+    const size_t sf = ((size_t)fz * ny0 + fy) * nx0 + fx;
+    if (u) __builtin_nontemporal_store(dw_f2v{o[0].I, o[1].I}, (dw_f2v*)(u + sf));
+    if (ex) __builtin_nontemporal_store(dw_f4v{o[0].ex.x, o[0].ex.y, o[1].ex.x, o[1].ex.y}, (dw_f4v*)(ex + sf));
+    if (ey) __builtin_nontemporal_store(dw_f4v{o[0].ey.x, o[0].ey.y, o[1].ey.x, o[1].ey.y}, (dw_f4v*)(ey + sf));
+    if (ez) __builtin_nontemporal_store(dw_f4v{o[0].ez.x, o[0].ez.y, o[1].ez.x, o[1].ez.y}, (dw_f4v*)(ez + sf));
   }
 }
 
@@ -645,6 +757,27 @@ static void launch_mirror(int nx0, int ny0, int nz0, const float* uo, const floa
   }
 }
 
+// The spread of every octant plane from the pair-indexed integral table
+// (16-byte aligned Ex/Ey/Ez, 8-byte aligned I: the caller checks).
+static void launch_spread_pairs(const DwArgs& a, int nx0, int ny0, int nz0, const float2* itab,
+                                const float4* trig, float* u, float2* ex, float2* ey, float2* ez,
+                                hipStream_t st) {
+  const int hz = nz0 / 2;
+  PlaneMap pm;
+  pm.hi0 = hz;         // images hz + o, o < hz
+  pm.n_hi = hz;
+  pm.lo0 = hz - 1;     // images hz - o, o >= 1
+  const int planes = nz0;
+  const int nxo = nx0 / 2 + 1;
+  // up to 8 octant rows per work-group within 48 KiB of LDS (the caller
+  // keeps nx0 <= 2048, so at least one row fits)
+  int rows = (int)((48 * 1024) / ((size_t)nxo * 3 * sizeof(float2)));
+  rows = rows < 1 ? 1 : (rows > 8 ? 8 : rows);
+  const size_t lds = (size_t)rows * nxo * 3 * sizeof(float2);
+  hipLaunchKernelGGL(k_debye_spread_pairs, dim3((nxo + rows - 1) / rows, 1, planes), dim3(256),
+                     lds, st, a, nx0, ny0, nz0, rows, itab, trig, u, ex, ey, ez, pm);
+}
+
 // Tables and product for the grid in `a` (octant or whole grid) on stream
 // st; with `full`, the octant is then spread to the full grid.
 static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st,
@@ -671,20 +804,36 @@ static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st
   const int nzt = (nz + 15) / 16, zp = nzt * 16;
   const int cp = (ntab + 16 * DM_CT - 1) / (16 * DM_CT) * (16 * DM_CT);
   const size_t nA = (size_t)3 * tp * cp, nP = (size_t)tp * zp;
+  // full grid from the pair table: the product stores the three integrals
+  // per (plane, pair) and the spread forms the fields (needs the 16-byte
+  // aligned outputs of the vector spread); otherwise the octant's fields
+  // are stored and mirrored
+  const bool itab_path = full && pairs && full->nx0 <= 2048 && ((uintptr_t)full->u % 8 == 0) &&
+                         ((uintptr_t)full->ex % 16 == 0) && ((uintptr_t)full->ey % 16 == 0) &&
+                         ((uintptr_t)full->ez % 16 == 0);
   const size_t noct = full ? (size_t)nx * ny * nz : 0;
+  const size_t out_bytes = !full ? 0
+                           : itab_path ? (size_t)3 * nz * ntab * sizeof(float2) +
+                                             (size_t)nx * ny * sizeof(float4)
+                                       : noct * 28;
   const size_t tab_bytes = nA * sizeof(float) + nP * sizeof(float2) + tp * sizeof(float) +
                            3 * (size_t)total * sizeof(float) + 256;
   void* scr = nullptr;
   hipMemPool_t pool = nullptr;
   hipError_t e = table_pool(&pool);
   if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(e));
-  e = hipMallocFromPoolAsync(&scr, tab_bytes + noct * 28 + 64, pool, st);
+  e = hipMallocFromPoolAsync(&scr, tab_bytes + out_bytes + 64, pool, st);
   if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_NOMEM, "stream-ordered allocation failed for the Debye tables");
   float* A = (float*)scr;
   float2* P = (float2*)(((uintptr_t)(A + nA) + 15) & ~(uintptr_t)15);
   float* co = (float*)(P + nP);
   float* par = co + tp;
-  if (full) {
+  float4* trig = nullptr;
+  if (itab_path) {
+    // integral table (3, nz, npairs), then the azimuth terms (16-byte aligned)
+    a.itab = (float2*)(((uintptr_t)scr + tab_bytes + 15) & ~(uintptr_t)15);
+    trig = (float4*)(a.itab + (size_t)3 * nz * ntab);
+  } else if (full) {
     // octant: Ex, Ey, Ez (16-byte aligned), then I
     float2* oex = (float2*)(((uintptr_t)scr + tab_bytes + 15) & ~(uintptr_t)15);
     a.ex = oex;
@@ -710,14 +859,23 @@ static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st
   const int wpg = dw_waves_per_group(nzt);
   const dim3 mgrid(cp / (16 * DM_CT), (nzt + wpg - 1) / wpg);
   if (!rc) rc = dw_timed(3, st, [&] {
-    if (pairs)
-      hipLaunchKernelGGL(k_dw_mfma<true>, mgrid, dim3(64 * wpg), 0, st, a, ntab, nzt, tp, cp,
+    if (itab_path)
+      hipLaunchKernelGGL((k_dw_mfma<true, true>), mgrid, dim3(64 * wpg), 0, st, a, ntab, nzt, tp, cp,
                          (const float2*)P, (const float*)A);
+    else if (pairs)
+      hipLaunchKernelGGL((k_dw_mfma<true, false>), mgrid, dim3(64 * wpg), 0, st, a, ntab, nzt, tp,
+                         cp, (const float2*)P, (const float*)A);
     else
-      hipLaunchKernelGGL(k_dw_mfma<false>, mgrid, dim3(64 * wpg), 0, st, a, ncols, nzt, tp, cp,
-                         (const float2*)P, (const float*)A);
+      hipLaunchKernelGGL((k_dw_mfma<false, false>), mgrid, dim3(64 * wpg), 0, st, a, ncols, nzt, tp,
+                         cp, (const float2*)P, (const float*)A);
   });
-  if (!rc && full)
+  if (!rc && itab_path)
+    rc = dw_timed(4, st, [&] {
+      hipLaunchKernelGGL(k_dw_trig, dim3((nx * ny + 255) / 256), dim3(256), 0, st, a, trig);
+      launch_spread_pairs(a, full->nx0, full->ny0, full->nz0, a.itab, trig, full->u, full->ex,
+                          full->ey, full->ez, st);
+    });
+  else if (!rc && full)
     rc = dw_timed(4, st, [&] {
       launch_mirror(full->nx0, full->ny0, full->nz0, a.I, a.ex, a.ey, a.ez, full->u, full->ex,
                     full->ey, full->ez, 0, nz, st);
