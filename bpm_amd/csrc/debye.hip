@@ -156,44 +156,13 @@ BPMH_DEV void combine(const DwArgs& a, const DwTrig& tr, float2 I0, float2 I1, f
 }
 
 // ----------------------------------------------------------------------------
-// grid path: sample table, phase table, tiled product
+// grid path: tables, tiled product, spread
 // ----------------------------------------------------------------------------
-// co[s] = cos t_s and A[n][s][col] = the three real amplitudes of sample s at
-// lateral point col = j*nx + i; the table is zero-padded to tp samples and
-// cp columns (A has row stride cp, plane stride tp*cp).
-// per-sample parameters (cos t, sin t, w) for the table below
-__global__ void __launch_bounds__(64) k_dw_params(DwArgs a, int total, float* par) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= total) return;
-  const int per_ann = a.S + 1;
-  const SampleParams p = sample_params(a, s / per_ann, s % per_ann);
-  par[3 * s] = p.co;
-  par[3 * s + 1] = p.si;
-  par[3 * s + 2] = p.w;
-}
-
-__global__ void __launch_bounds__(256) k_dw_samples(DwArgs a, int total, int ncols, int tp, int cp,
-                                                    const float* __restrict__ par, float* co, float* A) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = blockIdx.y;
-  if (col >= cp) return;
-  const size_t plane = (size_t)tp * cp;
-  const size_t o = (size_t)s * cp + col;
-  if (s >= total || col >= ncols) {
-    A[o] = A[plane + o] = A[2 * plane + o] = 0.f;
-    if (col == 0) co[s] = 0.f;
-    return;
-  }
-  const int i = col % a.nx, j = col / a.nx;
-  const float x = axis_coord(i, a.nx, a.x1, a.x2);
-  const float y = axis_coord(j, a.ny, a.y1, a.y2);
-  const float kr = k_times(a.lam, __fsqrt_rn(f_add(f_mul(x, x), f_mul(y, y))));
-  const Sample sm = sample_amps(SampleParams{par[3 * s], par[3 * s + 1], par[3 * s + 2]}, kr);
-  A[o] = sm.a0;
-  A[plane + o] = sm.a1;
-  A[2 * plane + o] = sm.a2;
-  if (col == 0) co[s] = sm.co;
-}
+// A[n][s][col] = the three real amplitudes of sample s at lateral point
+// col = j*nx + i, or at column pair col (below); P[s][z] = e^{i kz cos t_s}
+// with the float32 argument kz*cos t.  Both are zero-padded to tp samples,
+// cp columns and zp planes (A: row stride cp, plane stride tp*cp; P: row
+// stride zp).
 
 // Column pair p -> (i, j), i <= j, in the order (0,0), (0,1), (1,1), (0,2), ...
 BPMH_DEV void pair_ij(int p, int& i, int& j) {
@@ -209,44 +178,75 @@ BPMH_DEV void pair_ij(int p, int& i, int& j) {
 // rounds the same in either order), so column p holds the amplitudes of
 // both lateral points (i, j) and (j, i) of pair p; npairs = nx (nx + 1) / 2
 // columns, zero-padded to cp.
-__global__ void __launch_bounds__(256) k_dw_samples_pairs(DwArgs a, int total, int npairs, int tp, int cp,
-                                                          const float* __restrict__ par, float* co,
-                                                          float* A) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+
+// All tables of one grid call in a single launch, block row s = sample s:
+// blocks x < nbx fill the amplitudes A_n[s][col] (column pairs when PAIRS),
+// block nbx the phases P[s][z] = e^{i kz cos t_s}, and block nbx + 1 the
+// azimuth terms of the lateral points (when trig is set; points s*256 ..,
+// strided by the rows).  The sample's parameters (cos t, sin t, w) are
+// evaluated once per block; every value equals the one-kernel-per-table
+// form bit for bit.
+template <bool PAIRS>
+__global__ void __launch_bounds__(256) k_dw_tables(DwArgs a, int total, int ncols, int tp, int cp,
+                                                   int zp, float* __restrict__ A,
+                                                   float2* __restrict__ P,
+                                                   float4* __restrict__ trig) {
   const int s = blockIdx.y;
+  const int nbx = (cp + 255) / 256;
+  const int bx = blockIdx.x;
+  if (bx == nbx + 1) {
+    if (!trig) return;
+    for (int k = s * 256 + (int)threadIdx.x; k < a.nx * a.ny; k += tp * 256) {
+      const int i = k % a.nx, j = k / a.nx;
+      const float phi = (float)atan2((double)axis_coord(j, a.ny, a.y1, a.y2),
+                                     (double)axis_coord(i, a.nx, a.x1, a.x2));
+      const DwTrig tr = dw_trig(phi);
+      trig[k] = make_float4(tr.c2, tr.s2, tr.cp, tr.sp);
+    }
+    return;
+  }
+  __shared__ float sp[3];
+  if (threadIdx.x == 0) {
+    if (s < total) {
+      const int per_ann = a.S + 1;
+      const SampleParams p = sample_params(a, s / per_ann, s % per_ann);
+      sp[0] = p.co;
+      sp[1] = p.si;
+      sp[2] = p.w;
+    } else {
+      sp[0] = sp[1] = sp[2] = 0.f;
+    }
+  }
+  __syncthreads();
+  if (bx == nbx) {
+    for (int z = threadIdx.x; z < zp; z += blockDim.x) {
+      float2 v = make_float2(0.f, 0.f);
+      if (z < a.nz && s < total) {
+        const float kz = k_times(a.lam, axis_coord(z, a.nz, a.z1, a.z2));
+        sincosf(kz * sp[0], &v.y, &v.x);
+      }
+      P[(size_t)s * zp + z] = v;
+    }
+    return;
+  }
+  const int col = bx * 256 + (int)threadIdx.x;
   if (col >= cp) return;
   const size_t plane = (size_t)tp * cp;
   const size_t o = (size_t)s * cp + col;
-  if (s >= total || col >= npairs) {
+  if (s >= total || col >= ncols) {
     A[o] = A[plane + o] = A[2 * plane + o] = 0.f;
-    if (col == 0) co[s] = 0.f;
     return;
   }
   int i, j;
-  pair_ij(col, i, j);
+  if (PAIRS) pair_ij(col, i, j);
+  else i = col % a.nx, j = col / a.nx;
   const float x = axis_coord(i, a.nx, a.x1, a.x2);
   const float y = axis_coord(j, a.ny, a.y1, a.y2);
   const float kr = k_times(a.lam, __fsqrt_rn(f_add(f_mul(x, x), f_mul(y, y))));
-  const Sample sm = sample_amps(SampleParams{par[3 * s], par[3 * s + 1], par[3 * s + 2]}, kr);
+  const Sample sm = sample_amps(SampleParams{sp[0], sp[1], sp[2]}, kr);
   A[o] = sm.a0;
   A[plane + o] = sm.a1;
   A[2 * plane + o] = sm.a2;
-  if (col == 0) co[s] = sm.co;
-}
-
-// P[s][z] = e^{i kz cos t_s}, with the float32 argument kz*cos t; zero-padded
-// to tp samples and zp planes (row stride zp)
-__global__ void __launch_bounds__(64) k_dw_phases(DwArgs a, int total, int tp, int zp, const float* co,
-                                                  float2* P) {
-  const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = blockIdx.y;
-  if (z >= zp || s >= tp) return;
-  float2 v = make_float2(0.f, 0.f);
-  if (z < a.nz && s < total) {
-    const float kz = k_times(a.lam, axis_coord(z, a.nz, a.z1, a.z2));
-    sincosf(kz * co[s], &v.y, &v.x);
-  }
-  P[(size_t)s * zp + z] = v;
 }
 
 // MFMA form of the product (v_mfma_f32_16x16x4f32; wave64 operand layout:
@@ -417,18 +417,6 @@ __global__ void __launch_bounds__(256, 2) k_dw_mfma(DwArgs a, int ncols, int nzt
   dw_mfma_tiles<1, PAIRS, ITAB>(a, ncols, nzt * 16, tp, cp, zt0, c0, l, P, A);
 }
 
-// Azimuth terms of every lateral octant point (row-major, nx * ny):
-// dw_trig of atan2(y_j, x_i), the same values the product's own epilogue
-// uses for the point.
-__global__ void k_dw_trig(DwArgs a, float4* __restrict__ trig) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= a.nx * a.ny) return;
-  const int i = k % a.nx, j = k / a.nx;
-  const float phi = (float)atan2((double)axis_coord(j, a.ny, a.y1, a.y2),
-                                 (double)axis_coord(i, a.nx, a.x1, a.x2));
-  const DwTrig tr = dw_trig(phi);
-  trig[k] = make_float4(tr.c2, tr.s2, tr.cp, tr.sp);
-}
 
 __global__ void __launch_bounds__(64) k_debye_points(DwArgs a, int n) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -630,45 +618,80 @@ static int fill_args(DwArgs& a, float lam, const float* alphas, int n_alphas, in
   return BPMH_OK;
 }
 
-// The library's stream-ordered memory pool for the per-call Debye tables, one
-// per device, created on first use.  It keeps freed memory instead of
-// returning it to the driver, so repeated calls reuse the tables' pages
-// (with the HIP default of releasing at every synchronisation, re-mapping
-// ~110 MB per 256^3 call cost 0.47 -> 0.42 ms).  It holds up to the largest
-// table set used until the process ends.
-static hipError_t table_pool(hipMemPool_t* out) {
-  static std::mutex mu;
-  static hipMemPool_t pools[64] = {};
+// Scratch of the grid path (tables, integral table or octant): one
+// grow-only device buffer per device, reused in stream order.  A call that
+// runs on another stream than the buffer's last user first makes its stream
+// wait for that use (an event), so calls on several streams stay ordered
+// without host synchronisation; growing waits for the last use on the host
+// and reallocates (hipMalloc), which happens only when a call needs more than
+// any call before it on this device.  A stream-ordered pool allocation per
+// call (hipMallocFromPoolAsync) was measured here to stall the host until
+// the previous call had finished, leaving the GPU idle between calls.
+struct DwScratch {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipEvent_t last_use = nullptr;
+  hipStream_t last_stream = nullptr;
+  bool used = false;
+};
+
+static std::mutex g_dw_mu;
+static DwScratch g_dw_scr[64];
+
+// Locks g_dw_mu (released by dw_scratch_release) and returns a buffer of at
+// least `bytes` that stream st may use after its pending work.
+static int dw_scratch_acquire(size_t bytes, hipStream_t st, void** out, int* dev_out) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return e;
-  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
-  std::lock_guard<std::mutex> lock(mu);
-  if (!pools[dev]) {
-    hipMemPoolProps props;
-    memset(&props, 0, sizeof props);
-    props.allocType = hipMemAllocationTypePinned;
-    props.handleTypes = hipMemHandleTypeNone;
-    props.location.type = hipMemLocationTypeDevice;
-    props.location.id = dev;
-    hipMemPool_t pool = nullptr;
-    e = hipMemPoolCreate(&pool, &props);
-    if (e != hipSuccess) return e;
-    uint64_t keep = UINT64_MAX;
-    e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(e));
+  if (dev < 0 || dev >= 64) return bpmh_set_error(BPMH_ERR_HIP, "device index out of range");
+  g_dw_mu.lock();
+  DwScratch& d = g_dw_scr[dev];
+  if (!d.last_use) {
+    e = hipEventCreateWithFlags(&d.last_use, hipEventDisableTiming);
     if (e != hipSuccess) {
-      (void)hipMemPoolDestroy(pool);
-      return e;
+      g_dw_mu.unlock();
+      return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(e));
     }
-    pools[dev] = pool;
   }
-  *out = pools[dev];
-  return hipSuccess;
+  if (d.bytes < bytes) {
+    if (d.used) (void)hipEventSynchronize(d.last_use);
+    if (d.ptr) (void)hipFree(d.ptr);
+    d.ptr = nullptr;
+    d.bytes = 0;
+    e = hipMalloc(&d.ptr, bytes);
+    if (e != hipSuccess) {
+      d.ptr = nullptr;
+      g_dw_mu.unlock();
+      return bpmh_set_error(BPMH_ERR_NOMEM, "device allocation failed for the Debye tables");
+    }
+    d.bytes = bytes;
+    d.used = false;
+  } else if (d.used && d.last_stream != st) {
+    e = hipStreamWaitEvent(st, d.last_use, 0);
+    if (e != hipSuccess) {
+      g_dw_mu.unlock();
+      return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(e));
+    }
+  }
+  *out = d.ptr;
+  *dev_out = dev;
+  return BPMH_OK;
+}
+
+// Marks the buffer used by the work just enqueued on st and unlocks.
+static void dw_scratch_release(int dev, hipStream_t st) {
+  DwScratch& d = g_dw_scr[dev];
+  if (hipEventRecord(d.last_use, st) == hipSuccess) {
+    d.last_stream = st;
+    d.used = true;
+  }
+  g_dw_mu.unlock();
 }
 
 // Optional HIP-event timing of the grid path's kernels (this thread's calls):
-// kinds 0 parameters, 1 sample table, 2 phase table, 3 product, 4 full-grid
-// spread.
+// kinds 0 tables (sample parameters, amplitudes, phases, azimuth terms: one
+// launch), 1 product, 2 full-grid spread; 3 and 4 stay zero.
 struct DwTiming {
   bool on = false;
   std::vector<hipEvent_t> ev;  // pairs
@@ -816,18 +839,13 @@ static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st
                            : itab_path ? (size_t)3 * nz * ntab * sizeof(float2) +
                                              (size_t)nx * ny * sizeof(float4)
                                        : noct * 28;
-  const size_t tab_bytes = nA * sizeof(float) + nP * sizeof(float2) + tp * sizeof(float) +
-                           3 * (size_t)total * sizeof(float) + 256;
+  const size_t tab_bytes = nA * sizeof(float) + nP * sizeof(float2) + 256;
   void* scr = nullptr;
-  hipMemPool_t pool = nullptr;
-  hipError_t e = table_pool(&pool);
-  if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(e));
-  e = hipMallocFromPoolAsync(&scr, tab_bytes + out_bytes + 64, pool, st);
-  if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_NOMEM, "stream-ordered allocation failed for the Debye tables");
+  int sdev = 0;
+  const int arc = dw_scratch_acquire(tab_bytes + out_bytes + 64, st, &scr, &sdev);
+  if (arc) return arc;
   float* A = (float*)scr;
   float2* P = (float2*)(((uintptr_t)(A + nA) + 15) & ~(uintptr_t)15);
-  float* co = (float*)(P + nP);
-  float* par = co + tp;
   float4* trig = nullptr;
   if (itab_path) {
     // integral table (3, nz, npairs), then the azimuth terms (16-byte aligned)
@@ -842,23 +860,17 @@ static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st
     a.I = (float*)(oex + 3 * noct);
   }
   int rc = dw_timed(0, st, [&] {
-    hipLaunchKernelGGL(k_dw_params, dim3((total + 63) / 64), dim3(64), 0, st, a, total, par);
-  });
-  if (!rc) rc = dw_timed(1, st, [&] {
+    const dim3 tgrid((cp + 255) / 256 + 2, tp);
     if (pairs)
-      hipLaunchKernelGGL(k_dw_samples_pairs, dim3((cp + 255) / 256, tp), dim3(256), 0, st, a, total,
-                         ntab, tp, cp, (const float*)par, co, A);
+      hipLaunchKernelGGL(k_dw_tables<true>, tgrid, dim3(256), 0, st, a, total, ntab, tp, cp, zp, A,
+                         P, trig);
     else
-      hipLaunchKernelGGL(k_dw_samples, dim3((cp + 255) / 256, tp), dim3(256), 0, st, a, total, ncols,
-                         tp, cp, (const float*)par, co, A);
-  });
-  if (!rc) rc = dw_timed(2, st, [&] {
-    hipLaunchKernelGGL(k_dw_phases, dim3((zp + 63) / 64, tp), dim3(64), 0, st, a, total, tp, zp,
-                       (const float*)co, P);
+      hipLaunchKernelGGL(k_dw_tables<false>, tgrid, dim3(256), 0, st, a, total, ncols, tp, cp, zp, A,
+                         P, trig);
   });
   const int wpg = dw_waves_per_group(nzt);
   const dim3 mgrid(cp / (16 * DM_CT), (nzt + wpg - 1) / wpg);
-  if (!rc) rc = dw_timed(3, st, [&] {
+  if (!rc) rc = dw_timed(1, st, [&] {
     if (itab_path)
       hipLaunchKernelGGL((k_dw_mfma<true, true>), mgrid, dim3(64 * wpg), 0, st, a, ntab, nzt, tp, cp,
                          (const float2*)P, (const float*)A);
@@ -870,22 +882,18 @@ static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st
                          cp, (const float2*)P, (const float*)A);
   });
   if (!rc && itab_path)
-    rc = dw_timed(4, st, [&] {
-      hipLaunchKernelGGL(k_dw_trig, dim3((nx * ny + 255) / 256), dim3(256), 0, st, a, trig);
+    rc = dw_timed(2, st, [&] {
       launch_spread_pairs(a, full->nx0, full->ny0, full->nz0, a.itab, trig, full->u, full->ex,
                           full->ey, full->ez, st);
     });
   else if (!rc && full)
-    rc = dw_timed(4, st, [&] {
+    rc = dw_timed(2, st, [&] {
       launch_mirror(full->nx0, full->ny0, full->nz0, a.I, a.ex, a.ey, a.ez, full->u, full->ex,
                     full->ey, full->ez, 0, nz, st);
     });
-  if (rc) {
-    (void)hipFreeAsync(scr, st);
-    return rc;
-  }
-  e = hipGetLastError();
-  (void)hipFreeAsync(scr, st);
+  const hipError_t e = hipGetLastError();
+  dw_scratch_release(sdev, st);
+  if (rc) return rc;
   if (e != hipSuccess) return bpmh_set_error(BPMH_ERR_HIP, hipGetErrorString(e));
   return BPMH_OK;
 }

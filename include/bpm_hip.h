@@ -299,9 +299,10 @@ int bpmh_focus_field_debye(int nx0, int ny0, int nz0, double dx, double dy, doub
  * bpmh_focus_field_debye) on the calling thread: with timing on, every
  * kernel is bracketed by a HIP event pair; bpmh_debye_kernel_times waits for
  * them and returns the summed milliseconds and launch counts per kernel
- * kind since the last read-out in ms[5], counts[5] (0 sample parameters,
- * 1 sample table, 2 phase table, 3 product, 4 full-grid spread of
- * bpmh_focus_field_debye).  The reference only
+ * kind since the last read-out in ms[5], counts[5] (0 tables: the sample
+ * parameters, amplitudes, phases and azimuth terms in one launch, 1 product,
+ * 2 full-grid spread of bpmh_focus_field_debye; 3 and 4 stay zero).  The
+ * reference only
  * prints wall-clock deltas (bpm/psf/_focus_fields_debye.py:100,199,210). */
 int bpmh_debye_set_timing(int enable);
 int bpmh_debye_kernel_times(float* ms, int* counts);
