@@ -411,9 +411,18 @@ __global__ void __launch_bounds__(256, 2) k_dw_mfma(DwArgs a, int ncols, int nzt
                                                 const float2* __restrict__ P,
                                                 const float* __restrict__ A) {
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int zt0 = blockIdx.y * (int)(blockDim.x >> 6) + w;
-  if (zt0 >= nzt) return;  // whole wavefront; the kernel has no barriers
-  const int c0 = blockIdx.x * (16 * DM_CT);
+  const int wpg = (int)(blockDim.x >> 6);
+  // one-dimensional grid dealt round-robin over the 8 XCDs: XCD x takes the
+  // column blocks x*per .. x*per+per-1, the plane groups of a block one after
+  // another, so the work-groups reading a block's table columns share an L2
+  const int ncb = cp / (16 * DM_CT), ngz = (nzt + wpg - 1) / wpg;
+  const int per = (ncb + 7) / 8;
+  const int id = (int)blockIdx.x;
+  const int k = id >> 3;
+  const int cb = (id & 7) * per + k / ngz;
+  const int zt0 = (k % ngz) * wpg + w;
+  if (cb >= ncb || zt0 >= nzt) return;  // whole wavefront; the kernel has no barriers
+  const int c0 = cb * (16 * DM_CT);
   dw_mfma_tiles<1, PAIRS, ITAB>(a, ncols, nzt * 16, tp, cp, zt0, c0, l, P, A);
 }
 
@@ -807,12 +816,9 @@ static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st
                           const DwFull* full = nullptr) {
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int ncols = nx * ny;
-  // samples index the y extent of two grids (at most 65535); the product
-  // grid's y extent is the plane tiles over the wavefronts per work-group
+  // samples index the y extent of the tables grid (at most 65535)
   const long long total_ll = ((long long)int_steps + 1) * (n_alphas / 2);
-  const int nzt_chk = nz > (1 << 23) ? 0 : (nz + 15) / 16;
   if (total_ll > 65000 || nz > (1 << 23) ||
-      (nzt_chk + dw_waves_per_group(nzt_chk) - 1) / dw_waves_per_group(nzt_chk) > 65535 ||
       (long long)ncols * (total_ll + 64) > (1LL << 31) ||
       (long long)(nz + 16) * (total_ll + 64) > (1LL << 31))
     return bpmh_set_error(BPMH_ERR_INVALID, "grid x samples too large");
@@ -827,6 +833,11 @@ static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st
   const int nzt = (nz + 15) / 16, zp = nzt * 16;
   const int cp = (ntab + 16 * DM_CT - 1) / (16 * DM_CT) * (16 * DM_CT);
   const size_t nA = (size_t)3 * tp * cp, nP = (size_t)tp * zp;
+  // the product's one-dimensional grid: column blocks x plane groups, padded
+  // to a multiple of 8 XCDs
+  const int wpg = dw_waves_per_group(nzt);
+  const long long mblocks = 8LL * ((cp / (16 * DM_CT) + 7) / 8) * ((nzt + wpg - 1) / wpg);
+  if (mblocks > (1LL << 31) - 1) return bpmh_set_error(BPMH_ERR_INVALID, "grid x samples too large");
   // full grid from the pair table: the product stores the three integrals
   // per (plane, pair) and the spread forms the fields (needs the 16-byte
   // aligned outputs of the vector spread); otherwise the octant's fields
@@ -868,8 +879,7 @@ static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st
       hipLaunchKernelGGL(k_dw_tables<false>, tgrid, dim3(256), 0, st, a, total, ncols, tp, cp, zp, A,
                          P, trig);
   });
-  const int wpg = dw_waves_per_group(nzt);
-  const dim3 mgrid(cp / (16 * DM_CT), (nzt + wpg - 1) / wpg);
+  const dim3 mgrid((unsigned)mblocks);
   if (!rc) rc = dw_timed(1, st, [&] {
     if (itab_path)
       hipLaunchKernelGGL((k_dw_mfma<true, true>), mgrid, dim3(64 * wpg), 0, st, a, ntab, nzt, tp, cp,
