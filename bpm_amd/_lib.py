@@ -45,9 +45,9 @@ SIGNATURES = {
                                    _P]),
     "bpmh_adjoint_segment": (_I, [_P, _I, _I, _I, _P, _P, _LL, _P, _I, _LL, _P, _F, _I, _P, _P,
                                   _P]),
-    "bpmh_weighted_segment": (_I, [_P, _I, _I, _I, _P, _P, _LL, _F, _F, _F, _F, _F, _F, _I, _P,
-                                   _P, _P, _LL, _P]),
-    "bpmh_weighted_segment_work": (_I, [_P, _I, _I, _I, _I, _P]),
+    "bpmh_mean_segment": (_I, [_P, _I, _I, _I, _I, _P, _P, _LL, _P, _P, _F, _F, _F, _F, _F, _F,
+                               _I, _P, _P, _P, _LL, _P]),
+    "bpmh_mean_segment_work": (_I, [_P, _I, _I, _I, _I, _P]),
     "bpmh_floor_steps": (_I, [_P, _I, _I, _P, _P, _LL, _P]),
     "bpmh_debye_wolf": (_I, [_I, _I, _I, _F, _F, _F, _F, _F, _F, _F, _P, _I, _I, _F, _F, _I,
                              _F, _I, _P, _P, _P, _P, _P]),

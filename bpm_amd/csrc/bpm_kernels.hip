@@ -442,9 +442,9 @@ struct ColArgs {
   const float2* pw;        // plane-wave DC per plane (nz)
   float2* spart;           // scattering per-column partials (batch, nz, nx)
   int conj_h;              // adjoint: multiply by conj(H)
-  // weighted forward, recording for the adjoint (null: nothing recorded):
-  // PHASE 1 also stores U = F u of this step (item stride kout_bstride),
-  // PHASE 2 stores this step's dn0 and sum|U| per item
+  // forward recorded for the adjoint (null: nothing recorded): PHASE 1, and
+  // PHASE 0 before H, also store U = F u of this step (item stride
+  // kout_bstride); PHASE 2 stores this step's dn0 and sum|U| per item
   float2* kout;
   long long kout_bstride;
   float* rec_c;
@@ -458,6 +458,10 @@ struct ColArgs {
   float* s2part;           // (batch, nx): per column sum Im(conj(X) H U) / beta
   const float* s1part;     // (batch, ny): per row sum of the screen gradient
   float* gd_item;          // (batch, ny, nx): += dL/d(weight slice) per item
+  // simple mean (PHASE 3, 4): dn0 = mean(dn[z]) does not depend on the
+  // field: n = n_eff[z], no |U| term, and PHASE 4 stores dL/ddn0 per item
+  int simple;
+  float* cg_out;
 };
 
 // PHASE 0: FFT_y, H, IFFT_y.  PHASE 1: FFT_y, weighted partials, store.
@@ -465,7 +469,8 @@ struct ColArgs {
 // Adjoint of a weighted step (X = F G_v, the k-space gradient):
 // PHASE 3: FFT_y, per-column partials of sum Im(conj(X) H U)/beta, store X.
 // PHASE 4: dL/ddn0 from the partials, conj(H) X plus the term of dn0's
-//          dependence on |U|, the weight-slice gradient, IFFT_y.
+//          dependence on |U|, the weight-slice gradient, IFFT_y (simple mean:
+//          conj(H) X, IFFT_y, dL/ddn0 stored).
 template <int NY, int V, int PHASE, int EE>
 __global__ void __launch_bounds__((ColCfg<NY, V, EE>::THREADS))
 k_col(ColArgs a) {
@@ -572,7 +577,10 @@ k_col(ColArgs a) {
       }
     }
   }
-  if constexpr (ADJW) dn0w = a.cz[b];  // the forward's dn0 of this step
+  if constexpr (ADJW) dn0w = a.simple ? 0.f : a.cz[b];  // the forward's dn0 of this step
+  // the step's refractive index: weighted n0 + dn0 (as PHASE 2), simple n_eff[z]
+  float n_adj = 0.f;
+  if constexpr (ADJW) n_adj = a.simple ? a.n_eff[a.z] : f_add(a.n0, dn0w);
   float cgrad = 0.f, inv_w = 0.f, n_pts = 0.f;
   if constexpr (PHASE == 4) {
     // dL/ddn0 = -sum_x (screen gradient) + dz (n0+dn0) k0^2 / N sum_k Im(conj(X) H U)/beta
@@ -580,14 +588,27 @@ k_col(ColArgs a) {
     const float2 sums = group_sum_pair(a.s1part + (size_t)b * NY, NY, a.s2part + (size_t)b * a.nx,
                                        a.nx, cg_slot);
     n_pts = (float)a.nx * (float)NY;
-    const float n = a.n0 + dn0w;
-    cgrad = -sums.x + a.dz * n * a.k0 * a.k0 / n_pts * sums.y;
-    inv_w = 1.f / a.wz[b];
+    cgrad = -sums.x + a.dz * n_adj * a.k0 * a.k0 / n_pts * sums.y;
+    if (a.simple) {
+      if (bx == 0 && threadIdx.x == 0) a.cg_out[b] = cgrad;
+    } else {
+      inv_w = 1.f / a.wz[b];
+    }
   }
 
   if constexpr (FWD) {
 #pragma unroll
     for (int q = 0; q < P; ++q) line_fft<NY, E, false, CL>(v[q], lds[q * LC + c], t, tw);
+  }
+  if constexpr (PHASE == 0) {
+    // forward recorded for the adjoint: U = F u of this step, before H
+    if (a.kout) {
+      float2* ko = a.kout + (size_t)b * a.kout_bstride + x0;
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+#pragma unroll
+        for (int m = 0; m < E; ++m) ko[(size_t)(t + m * T) * a.nx + q] = v[q][m];
+    }
   }
 
 #pragma unroll
@@ -609,7 +630,7 @@ k_col(ColArgs a) {
       if (t == 0) a.wpart[(size_t)b * a.nx + x] = acc;
     } else if constexpr (PHASE == 3) {
       // dH/ddn0 = H * (-i dz (n0+dn0) k0^2 / beta): collect Im(conj(X) H U)/beta
-      const float an = n_term_f32(f_add(a.n0, dn0w), a.k0);
+      const float an = n_term_f32(n_adj, a.k0);
       const float kx2 = kcomp2_f32(x, a.nx, a.dx);
       float acc = 0.f;
 #pragma unroll
@@ -627,7 +648,7 @@ k_col(ColArgs a) {
     } else {
       float an = 0.f, kx2 = 0.f;
       if (a.hmode != 0) {
-        float n = (a.hmode == 1) ? a.n_eff[a.z] : f_add(a.n0, dn0w);
+        float n = ADJW ? n_adj : (a.hmode == 1) ? a.n_eff[a.z] : f_add(a.n0, dn0w);
         an = n_term_f32(n, a.k0);
         kx2 = kcomp2_f32(x, a.nx, a.dx);
       }
@@ -638,7 +659,7 @@ k_col(ColArgs a) {
             (a.hmode == 0) ? hv[q][m] : prop_f32(kx2, kcomp2_f32(ky, NY, a.dy), an, a.dz);
         v[q][m] = a.conj_h ? c_mulc(v[q][m], h) : c_mul(v[q][m], h);
       }
-      if constexpr (PHASE == 4) {
+      if (PHASE == 4 && !a.simple) {
         // dn0 = sum|U| d / sum|U| depends on U through |U|:
         // G_U += dL/ddn0 (d - dn0) / W * U/|U| (times N: X is N G_V), and
         // dL/dd += dL/ddn0 |U| / W
@@ -1997,7 +2018,7 @@ extern "C" int bpmh_adjoint_segment(bpmh_plan plan, int batch, int z_lo, int z_h
 }
 
 // ----------------------------------------------------------------------------
-// weighted-mean segment of the differentiable form
+// mean-corrected (weighted or simple) segment of the differentiable form
 // ----------------------------------------------------------------------------
 // out[r] = sum of row r (nx floats) of g; one wavefront per row, fixed order
 __global__ void __launch_bounds__(256) k_row_sums(const float* __restrict__ g,
@@ -2018,9 +2039,9 @@ __global__ void k_fill_zero(float* __restrict__ p, size_t n) {
   if (i < n) p[i] = 0.f;
 }
 
-// Workspace of bpmh_weighted_segment, in floats (each part 256-byte aligned).
+// Workspace of bpmh_mean_segment, in floats (each part 256-byte aligned).
 struct WsegLayout {
-  size_t wpart, dn0, fplane, useg, kseg, recc, recw, s1, s2, gitem, gd, total;
+  size_t wpart, dn0, fplane, useg, kseg, recc, recw, s1, s2, gitem, gd, cg, total;
 };
 
 static WsegLayout wseg_layout(int nx, int ny, int batch, int nseg, bool adjoint) {
@@ -2045,25 +2066,42 @@ static WsegLayout wseg_layout(int nx, int ny, int batch, int nseg, bool adjoint)
     L.s2 = take(B * nx);
     L.gitem = take(B * n2);
     L.gd = take(B * n2);
+    L.cg = take(B);
   }
   L.total = o;
   return L;
 }
 
-extern "C" int bpmh_weighted_segment_work(bpmh_plan plan, int batch, int z_lo, int z_hi,
-                                          int adjoint, long long* numel) {
+// grad[i] += (sum_b cg[b]) / n over one plane: dL/ddn0 of a simple-mean step
+// spread over the plane it is the mean of (items summed in order)
+__global__ void k_add_mean_grad(float* __restrict__ grad, const float* __restrict__ cg, int batch,
+                                size_t n, float inv_n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float c = 0.f;
+  for (int b = 0; b < batch; ++b) c += cg[b];
+  grad[i] += c * inv_n;
+}
+
+extern "C" int bpmh_mean_segment_work(bpmh_plan plan, int batch, int z_lo, int z_hi,
+                                      int adjoint, long long* numel) {
   if (!plan || !numel) return bpmh_set_error(BPMH_ERR_INVALID, "null argument");
   if (batch < 1 || z_lo < 0 || z_hi < z_lo) return bpmh_set_error(BPMH_ERR_INVALID, "bad range");
   *numel = (long long)wseg_layout(plan->nx, plan->ny, batch, z_hi - z_lo + 1, adjoint != 0).total;
   return BPMH_OK;
 }
 
-extern "C" int bpmh_weighted_segment(bpmh_plan plan, int batch, int z_lo, int z_hi, float* plane_,
-                                     const float* dn, long long dn_numel, float n0, float k0,
-                                     float dx, float dy, float dz, float unit_k, int absorb,
-                                     float* g_, float* grad_dn, float* work, long long work_numel,
-                                     void* stream) {
+extern "C" int bpmh_mean_segment(bpmh_plan plan, int mean_mode, int batch, int z_lo, int z_hi,
+                                 float* plane_, const float* dn, long long dn_numel,
+                                 const float* n_eff, const float* dn0_arr, float n0, float k0,
+                                 float dx, float dy, float dz, float unit_k, int absorb, float* g_,
+                                 float* grad_dn, float* work, long long work_numel, void* stream) {
   if (!plan || !plane_ || !dn || !work) return bpmh_set_error(BPMH_ERR_INVALID, "null argument");
+  if (mean_mode != BPMH_MEAN_SIMPLE && mean_mode != BPMH_MEAN_WEIGHTED)
+    return bpmh_set_error(BPMH_ERR_INVALID, "mean_mode must be simple or weighted");
+  const bool simple = mean_mode == BPMH_MEAN_SIMPLE;
+  if (simple && (!n_eff || !dn0_arr))
+    return bpmh_set_error(BPMH_ERR_INVALID, "mean_mode simple needs n_eff and dn0 arrays");
   const bool adjoint = g_ != nullptr;
   if (adjoint && !grad_dn) return bpmh_set_error(BPMH_ERR_INVALID, "the adjoint needs grad_dn");
   if (batch < 1 || z_lo < 0 || z_hi < z_lo) return bpmh_set_error(BPMH_ERR_INVALID, "bad range");
@@ -2093,6 +2131,7 @@ extern "C" int bpmh_weighted_segment(bpmh_plan plan, int batch, int z_lo, int z_
   float2* plane = (float2*)plane_;
   float2* fp = adjoint ? (float2*)(work + L.fplane) : plane;
   const int cv = col_variant_for(plan->col_v, fp);
+  const col_fn col0 = pick_col<0>(ny, cv);
   const col_fn colA = pick_col<1>(ny, cv);
   const col_fn colB = pick_col<2>(ny, cv);
   const size_t B = (size_t)batch, S = (size_t)nseg;
@@ -2114,8 +2153,9 @@ extern "C" int bpmh_weighted_segment(bpmh_plan plan, int batch, int z_lo, int z_
   ra.unit_k = unit_k;
   ra.inv_n = 1.0f / ((float)nx * (float)ny);
   ra.absorb = absorb;
-  ra.dn0_mode = BPMH_MEAN_WEIGHTED;
+  ra.dn0_mode = mean_mode;
   ra.dn0_item = work + L.dn0;
+  ra.dn0_arr = simple ? dn0_arr + z_lo : nullptr;  // per plane, indexed by the segment's z
   ra.out_field = adjoint ? useg : nullptr;
 
   ColArgs ca;
@@ -2124,7 +2164,8 @@ extern "C" int bpmh_weighted_segment(bpmh_plan plan, int batch, int z_lo, int z_
   ca.tw = plan->tw_y[cv / kColShapes];
   ca.nx = nx;
   ca.nz = nseg;
-  ca.hmode = BPMH_MEAN_WEIGHTED;
+  ca.hmode = mean_mode;
+  ca.n_eff = simple ? n_eff + z_lo : nullptr;
   ca.n0 = n0;
   ca.k0 = k0;
   ca.dx = dx;
@@ -2148,8 +2189,12 @@ extern "C" int bpmh_weighted_segment(bpmh_plan plan, int batch, int z_lo, int z_
       ca.rec_c = work + L.recc + (size_t)(z - 1) * B;
       ca.rec_w = work + L.recw + (size_t)(z - 1) * B;
     }
-    colA(ca, batch, s);
-    colB(ca, batch, s);
+    if (simple) {
+      col0(ca, batch, s);
+    } else {
+      colA(ca, batch, s);
+      colB(ca, batch, s);
+    }
     ra.z = z;
     ra.fwd_items = z < nseg - 1 ? batch : 0;
     row(ra, batch, s);
@@ -2183,7 +2228,10 @@ extern "C" int bpmh_weighted_segment(bpmh_plan plan, int batch, int z_lo, int z_
   cb.tw = ca.tw;
   cb.nx = nx;
   cb.nz = nseg;
-  cb.hmode = BPMH_MEAN_WEIGHTED;
+  cb.hmode = mean_mode;
+  cb.simple = simple;
+  cb.n_eff = simple ? n_eff + z_lo : nullptr;
+  cb.cg_out = work + L.cg;
   cb.conj_h = 1;
   cb.n0 = n0;
   cb.k0 = k0;
@@ -2201,7 +2249,7 @@ extern "C" int bpmh_weighted_segment(bpmh_plan plan, int batch, int z_lo, int z_
     rb.z = z;
     rb.useg_z = z;
     rb.inverse = z != nseg - 1;
-    rb.dn0_item = work + L.recc + (size_t)(z - 1) * B;
+    if (!simple) rb.dn0_item = work + L.recc + (size_t)(z - 1) * B;
     row(rb, batch, s);
     hipLaunchKernelGGL(k_grad_reduce, dim3(red_blocks), dim3(256), 0, s, gitem,
                        grad_dn + (size_t)(z_lo + z) * nn, batch, nn);
@@ -2213,6 +2261,10 @@ extern "C" int bpmh_weighted_segment(bpmh_plan plan, int batch, int z_lo, int z_
     cb.wz = work + L.recw + (size_t)(z - 1) * B;
     col3(cb, batch, s);
     col4(cb, batch, s);
+    if (simple)  // dn0 of this step is the mean of plane z: dL/ddn0 / N on every voxel
+      hipLaunchKernelGGL(k_add_mean_grad, dim3(red_blocks), dim3(256), 0, s,
+                         grad_dn + (size_t)(z_lo + z) * nn, (const float*)(work + L.cg), batch, nn,
+                         1.f / (float)nn);
   }
   rb.adjoint = 0;
   rb.screen = 0;
@@ -2220,7 +2272,8 @@ extern "C" int bpmh_weighted_segment(bpmh_plan plan, int batch, int z_lo, int z_
   rb.fwd_items = 0;
   row(rb, batch, s);
   // the weight slice is slice 0 of the volume
-  hipLaunchKernelGGL(k_grad_reduce, dim3(red_blocks), dim3(256), 0, s, gd, grad_dn, batch, nn);
+  if (!simple)
+    hipLaunchKernelGGL(k_grad_reduce, dim3(red_blocks), dim3(256), 0, s, gd, grad_dn, batch, nn);
   HIP_TRY(hipGetLastError());
   return BPMH_OK;
 }

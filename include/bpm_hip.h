@@ -236,32 +236,37 @@ int bpmh_adjoint_segment(bpmh_plan plan, int batch, int z_lo, int z_hi, float* g
                          long long dn_numel, const float* H, float unit_k, int absorb,
                          float* grad_dn, float* gitem, void* stream);
 
-/* The differentiable form under bpm_3d's default mean_mode "weighted"
- * (bpm/bpm/bpm_3d.py:149-159,302-311: dn0 = sum|U| d / sum|U| over the
- * k-space plane U = F u of each step, with d the volume's slice 0 read at
- * the same flat index; H(n0 + dn0) and the screen exp(-i k0 dz (dn - dn0))),
- * one segment z_lo..z_hi of a float32 dn volume (nz, ny, nx) shared by the
- * batch (dn_numel elements, at least (z_hi+1)*ny*nx); n0, k0, dx, dy, dz,
- * unit_k, absorb as for bpmh_propagate.
+/* The differentiable form under bpm_3d's mean corrections, one segment
+ * z_lo..z_hi of a float32 dn volume (nz, ny, nx) shared by the batch
+ * (dn_numel elements, at least (z_hi+1)*ny*nx); n0, k0, dx, dy, dz, unit_k,
+ * absorb as for bpmh_propagate.
+ * mean_mode BPMH_MEAN_WEIGHTED (bpm_3d's default, bpm/bpm/bpm_3d.py:149-159,
+ *   302-311): dn0 = sum|U| d / sum|U| over the k-space plane U = F u of each
+ *   step, with d the volume's slice 0 read at the same flat index;
+ * mean_mode BPMH_MEAN_SIMPLE (bpm_3d.py:216-217,288-293): dn0 = mean(dn[z])
+ *   of the step's target plane, given as the float32 arrays n_eff (n0 + dn0)
+ *   and dn0 over all nz planes, as for bpmh_propagate.
+ * Each step applies H(n0 + dn0) and the screen exp(-i k0 dz (dn - dn0)).
  * g == NULL: forward -- plane (batch, ny, nx) c64 is advanced in place from
  *   plane z_lo to plane z_hi, bit-identical to those steps of bpmh_propagate
- *   with mean_mode WEIGHTED over the whole volume.
+ *   in the same mean mode over the whole volume.
  * g != NULL: adjoint -- plane holds the forward plane z_lo and is only read;
- *   the segment is recomputed into work (keeping each step's U, dn0 and
- *   sum|U|), then g (batch, ny, nx) c64 is carried from G_{z_hi} = 2 dL/du*
- *   of plane z_hi to G_{z_lo}, and grad_dn (nz, ny, nx) f32 receives += dL/ddn
- *   for planes z_lo+1..z_hi (through the screens) and for slice 0 (through
+ *   the segment is recomputed into work (keeping each step's U and, when
+ *   weighted, its dn0 and sum|U|), then g (batch, ny, nx) c64 is carried from
+ *   G_{z_hi} = 2 dL/du* of plane z_hi to G_{z_lo}, and grad_dn (nz, ny, nx)
+ *   f32 receives += dL/ddn for planes z_lo+1..z_hi (through the screens and,
+ *   simple, through each plane's mean) and, weighted, for slice 0 (through
  *   the mean's weights).
- * work: device floats, at least what bpmh_weighted_segment_work reports
+ * work: device floats, at least what bpmh_mean_segment_work reports
  * (forward: 2*batch*nx + batch; adjoint: about batch*ny*nx*(4*nseg + 4)).
  * Nothing is allocated; calls sharing a work buffer must not overlap.
  * No reference counterpart (the reference has no gradient). */
-int bpmh_weighted_segment(bpmh_plan plan, int batch, int z_lo, int z_hi, float* plane,
-                          const float* dn, long long dn_numel, float n0, float k0, float dx,
-                          float dy, float dz, float unit_k, int absorb, float* g, float* grad_dn,
-                          float* work, long long work_numel, void* stream);
-int bpmh_weighted_segment_work(bpmh_plan plan, int batch, int z_lo, int z_hi, int adjoint,
-                               long long* numel);
+int bpmh_mean_segment(bpmh_plan plan, int mean_mode, int batch, int z_lo, int z_hi, float* plane,
+                      const float* dn, long long dn_numel, const float* n_eff, const float* dn0,
+                      float n0, float k0, float dx, float dy, float dz, float unit_k, int absorb,
+                      float* g, float* grad_dn, float* work, long long work_numel, void* stream);
+int bpmh_mean_segment_work(bpmh_plan plan, int batch, int z_lo, int z_hi, int adjoint,
+                           long long* numel);
 
 /* Debye-Wolf kernel variants */
 #define BPMH_DW_INDEX 0  /* debye_wolf, bpm/psf/kernels/psf_debye.cl:24-113 */

@@ -185,32 +185,37 @@ def test_ragged_batch_arguments_checked_before_launch(lib):
         assert lib.bpmh_plan_destroy(h) == 0
 
 
-def test_weighted_segment_and_floor_checked_before_launch(lib):
-    """bpmh_weighted_segment refuses a dn volume short of plane z_hi, a work
-    buffer below what bpmh_weighted_segment_work reports, and an adjoint
-    call without grad_dn; bpmh_floor_steps refuses a dn of fewer than nz
-    planes -- all before touching a device."""
+def test_mean_segment_and_floor_checked_before_launch(lib):
+    """bpmh_mean_segment refuses a dn volume short of plane z_hi, a work
+    buffer below what bpmh_mean_segment_work reports, an adjoint call
+    without grad_dn, the simple mean without its arrays and an unknown mean
+    mode; bpmh_floor_steps refuses a dn of fewer than nz planes -- all before
+    touching a device."""
     h = ctypes.c_void_p()
     assert lib.bpmh_plan_create(64, 32, ctypes.byref(h)) == 0
     try:
         n = 64 * 32
         fake = ctypes.c_void_p(4096)
         need = ctypes.c_longlong()
-        assert lib.bpmh_weighted_segment_work(h, 2, 0, 8, 0, ctypes.byref(need)) == 0
+        assert lib.bpmh_mean_segment_work(h, 2, 0, 8, 0, ctypes.byref(need)) == 0
         assert need.value == 256 + 64  # 2*batch*nx partials + batch dn0, 256-byte aligned parts
-        assert lib.bpmh_weighted_segment_work(h, 2, 0, 8, 1, ctypes.byref(need)) == 0
+        assert lib.bpmh_mean_segment_work(h, 2, 0, 8, 1, ctypes.byref(need)) == 0
         adj = need.value
         # recorded planes (9), k-space planes (8), the forward copy (c64) and two f32 planes
         assert adj >= 2 * 2 * n * (9 + 8 + 1) + 2 * 2 * n
 
-        def seg(z_hi, dn_numel, g, grad, work_numel):
-            return lib.bpmh_weighted_segment(h, 2, 0, z_hi, fake, fake, dn_numel, 1.33, 12.5, .1, .1,
-                                             .1, 1.25, 0, g, grad, fake, work_numel, None)
+        def seg(z_hi, dn_numel, g, grad, work_numel, mode=_lib.MEAN_WEIGHTED, arrays=None):
+            return lib.bpmh_mean_segment(h, mode, 2, 0, z_hi, fake, fake, dn_numel, arrays, arrays,
+                                         1.33, 12.5, .1, .1, .1, 1.25, 0, g, grad, fake,
+                                         work_numel, None)
 
         cases = [
             (lambda: seg(8, 8 * n, None, None, 10 ** 9), b"dn holds"),
             (lambda: seg(8, 9 * n, fake, fake, adj - 1), b"work holds"),
             (lambda: seg(8, 9 * n, fake, None, adj), b"needs grad_dn"),
+            (lambda: seg(8, 9 * n, None, None, adj, mode=_lib.MEAN_SIMPLE), b"needs n_eff"),
+            (lambda: seg(8, 9 * n, None, None, adj, mode=_lib.MEAN_IGNORE, arrays=fake),
+             b"simple or weighted"),
             (lambda: lib.bpmh_floor_steps(h, 1, 4, fake, fake, 3 * n, None), b"fewer than nz"),
         ]
         for fn, msg in cases:

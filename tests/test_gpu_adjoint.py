@@ -179,11 +179,12 @@ def test_adjoint_full_size_vs_finite_differences():
     assert err < 1e-3
 
 
-def reference_weighted_f64(dn, u0, units, lam, n0, absorbing_width=0):
-    """bpm_3d's default mean_mode "weighted" in complex128 with autograd: per
-    step U = FFT2(u), dn0 = sum|U| d / sum|U| with d = dn[0] read at U's
-    flat index (bpm/bpm/bpm_3d.py:149-159,302-311), then
-    u = exp(-i k0 dz (dn_z - dn0)) T * IFFT2(H(n0 + dn0) U) with
+def reference_mean_f64(dn, u0, units, lam, n0, absorbing_width=0, mode="weighted"):
+    """bpm_3d's mean corrections in complex128 with autograd.  Weighted (the
+    default): per step U = FFT2(u), dn0 = sum|U| d / sum|U| with d = dn[0]
+    read at U's flat index (bpm/bpm/bpm_3d.py:149-159,302-311); simple:
+    dn0 = mean(dn_z) of the step's target plane (bpm_3d.py:216-217,288-293).
+    Then u = exp(-i k0 dz (dn_z - dn0)) T * IFFT2(H(n0 + dn0) U) with
     H(n) = exp(-i dz sqrt(n^2 k0^2 - kx^2 - ky^2)) (0 where negative)."""
     import torch as t
     nz, ny, nx = dn.shape
@@ -201,8 +202,11 @@ def reference_weighted_f64(dn, u0, units, lam, n0, absorbing_width=0):
     u = u0.to(t.complex128)
     for z in range(1, nz):
         U = t.fft.fft2(u)
-        a = U.abs()
-        dn0 = (a * dn[0]).sum((-2, -1)) / a.sum((-2, -1))
+        if mode == "weighted":
+            a = U.abs()
+            dn0 = (a * dn[0]).sum((-2, -1)) / a.sum((-2, -1))
+        else:
+            dn0 = dn[z].mean().expand(u.shape[0])
         tmp = ((n0 + dn0) ** 2 * k0 ** 2)[:, None, None] - k2
         beta = t.sqrt(t.clamp(tmp, min=1e-30))
         H = t.where(tmp > 0, t.exp(-1j * dz * beta), t.zeros_like(beta, dtype=t.complex128))
@@ -241,7 +245,7 @@ def test_weighted_adjoint_vs_float64_autograd(absorb):
     loss.backward()
     dn_r = dn.double().clone().requires_grad_(True)
     u0_r = u0.to(t.complex128).clone().requires_grad_(True)
-    ref = reference_weighted_f64(dn_r, u0_r, (.1, .1, .1), LAM, N0, absorb)
+    ref = reference_mean_f64(dn_r, u0_r, (.1, .1, .1), LAM, N0, absorb)
     loss_r = (ref - target.to(t.complex128)).abs().pow(2).sum()
     loss_r.backward()
     g, gr = dn_a.grad.double(), dn_r.grad
@@ -262,3 +266,43 @@ def test_weighted_adjoint_vs_float64_autograd(absorb):
     (out_b - target).abs().pow(2).sum().backward()
     assert t.equal(dn_b.grad[1:], dn_a.grad[1:])
     assert ((dn_b.grad[0] - dn_a.grad[0]).norm() / dn_a.grad[0].norm()).item() < 1e-5
+
+
+def test_simple_mean_adjoint_vs_float64_autograd():
+    """mean_mode "simple" (dn0 = mean of the step's target plane): the
+    checkpointed forward equals bpm_3d's simple propagation bit for bit, and
+    loss, dL/ddn (the mean's share included: dL/ddn0 / N on every voxel of
+    the plane) and dL/du0 match float64 autograd of the same map."""
+    import torch as t
+    from bpm_amd import adjoint, bpm3d as B, _lib
+    nz, ny, nx, Bn = 18, 32, 64, 2
+    say("case simple-mean adjoint vs f64 autograd")
+    dn, dn2, u0 = setup(nz, ny, nx, Bn, seed=9)
+    target = adjoint.propagate(dn2, u0, lam=LAM, n0=N0, mean_mode="simple").detach()
+    # forward == bpmh_propagate with the same per-plane arrays
+    n_eff, dn0 = adjoint._simple_means(dn, N0)
+    plane = u0.clone()
+    B.propagate_fields(plane, nz, (.1, .1, .1), LAM, N0, dn=dn, dn_kind=_lib.DN_F32,
+                       mean_mode="simple", n_eff=n_eff, dn0=dn0)
+    out0 = adjoint.propagate(dn, u0, lam=LAM, n0=N0, checkpoint=5, mean_mode="simple")
+    assert t.equal(out0, plane), "checkpointed simple forward differs from bpmh_propagate"
+    dn_a = dn.clone().requires_grad_(True)
+    u0_a = u0.clone().requires_grad_(True)
+    out = adjoint.propagate(dn_a, u0_a, lam=LAM, n0=N0, checkpoint=5, mean_mode="simple")
+    loss = (out - target).abs().pow(2).sum()
+    loss.backward()
+    dn_r = dn.double().clone().requires_grad_(True)
+    u0_r = u0.to(t.complex128).clone().requires_grad_(True)
+    ref = reference_mean_f64(dn_r, u0_r, (.1, .1, .1), LAM, N0, mode="simple")
+    loss_r = (ref - target.to(t.complex128)).abs().pow(2).sum()
+    loss_r.backward()
+    g, gr = dn_a.grad.double(), dn_r.grad
+    e = ((g - gr).norm() / gr.norm()).item()
+    eu = ((u0_a.grad.to(t.complex128) - u0_r.grad).norm() / u0_r.grad.norm()).item()
+    # the mean's share: the plane-average of the gradient, against autograd's
+    em = ((g.mean(dim=(1, 2)) - gr.mean(dim=(1, 2))).norm() / gr.mean(dim=(1, 2)).norm()).item()
+    say("   loss %.6e vs f64 %.6e; grad_dn rel-L2 %.2e (plane means %.2e); grad_u0 rel-L2 %.2e"
+        % (loss.item(), loss_r.item(), e, em, eu))
+    assert abs(loss.item() - loss_r.item()) / loss_r.item() < 1e-4
+    assert e < 1e-4 and eu < 1e-4 and em < 1e-3
+    assert g[0].abs().max().item() == 0  # dn[0] enters no step
