@@ -4,8 +4,10 @@ A job is one dn volume (N x N x depth, random spheres generated on the GPU)
 illuminated by B tilted plane waves; its result is the B exit fields and
 summed intensities, copied back to the host.  The server
 
-* admits queued jobs (FIFO) whenever an execution slot is free and the job's
-  HBM footprint fits the remaining budget;
+* admits queued jobs whenever an execution slot is free and the job's HBM
+  footprint fits the remaining budget -- in arrival order (``policy="fifo"``),
+  or smallest work first (``"sjf"``, work = voxel-steps) except that a job
+  waiting longer than ``max_wait`` seconds goes first, in arrival order;
 * runs each admitted job on its slot's HIP stream with the slot's own plan
   (a plan's scratch and captured graphs serve one stream at a time), its B
   angles batched into one launch sequence (``bpmh_propagate`` with batch=B
@@ -29,8 +31,8 @@ from . import bpm3d as B
 from .spheres import spheres_dn_device
 from .workloads import small_workload_spheres
 
-__all__ = ["Job", "BpmServer", "poisson_jobs", "plain_loop", "job_footprint", "job_spheres",
-           "job_u0"]
+__all__ = ["Job", "BpmServer", "poisson_jobs", "plain_loop", "job_footprint", "job_work",
+           "job_spheres", "job_u0"]
 
 Job = collections.namedtuple("Job", "jid arrival N nz angles seed")
 
@@ -50,6 +52,11 @@ def poisson_jobs(seed=0, n_jobs=24, rate_hz=40., grids=(128, 256, 512, 1024), de
         th = rng.uniform(-max_tilt, max_tilt, (B_, 2)) / math.sqrt(2.)
         jobs.append(Job(j, t, N, nz, th, int(rng.integers(1 << 30))))
     return jobs
+
+
+def job_work(job):
+    """Voxel-steps of a job: N^2 x (depth - 1) x angles."""
+    return job.N * job.N * (job.nz - 1) * len(job.angles)
 
 
 def job_footprint(job):
@@ -140,10 +147,15 @@ class BpmServer:
     as its own planes are final."""
 
     def __init__(self, hbm_budget, slots=4, units=(.1, .1, .1), lam=.5, n0=1.33, fuse=True,
-                 max_group_jobs=16, max_group_items=256, serve_batch=16):
+                 max_group_jobs=16, max_group_items=256, serve_batch=16, policy="fifo",
+                 max_wait=1.0):
+        if policy not in ("fifo", "sjf"):
+            raise ValueError("policy must be 'fifo' or 'sjf'")
         self.budget = int(hbm_budget)
         self.slots = int(slots)
         self.fuse = bool(fuse)
+        self.policy = policy
+        self.max_wait = float(max_wait)
         self.max_group_jobs = max(1, min(16, int(max_group_jobs)))
         self.max_group_items = int(max_group_items)
         self.ctx = _Ctx(units, lam, n0, self.slots, serve_batch)
@@ -191,17 +203,27 @@ class BpmServer:
         # the device tensors stay alive until every job of the launch is retired
         return handles, (dn, plane, acc, u0s)
 
-    def _take_group(self, queue, used):
-        """The FIFO head job plus (when fusing) the later queued jobs of its
-        grid size that fit the group limits and the HBM budget; None when the
-        head does not fit yet."""
-        head = queue[0]
+    def _order(self, queue, now):
+        """Queued jobs in admission order under the server's policy."""
+        if self.policy == "fifo":
+            return list(queue)
+        late = [j for j in queue if now - j.arrival > self.max_wait]
+        rest = sorted((j for j in queue if now - j.arrival <= self.max_wait),
+                      key=lambda j: (job_work(j), j.arrival))
+        return late + rest
+
+    def _take_group(self, queue, used, now=0.):
+        """The first job in admission order plus (when fusing) the later
+        queued jobs of its grid size that fit the group limits and the HBM
+        budget; None when that job does not fit yet."""
+        order = self._order(queue, now)
+        head = order[0]
         if used + _group_footprint([head]) > self.budget:
             return None
         js = [head]
         if self.fuse:
             items = len(head.angles)
-            for j in list(queue)[1:]:
+            for j in order[1:]:
                 if len(js) >= self.max_group_jobs:
                     break
                 if j.N != head.N or items + len(j.angles) > self.max_group_items:
@@ -245,9 +267,10 @@ class BpmServer:
             now = clock() if realtime else float("inf")
             while pending and pending[0].arrival <= now:
                 queue.append(pending.pop(0))
-            # admit: FIFO head (and its same-grid followers) while a slot is free
+            # admit: the first job in policy order (and same-grid followers)
+            # while a slot is free
             while queue and free:
-                js = self._take_group(queue, used)
+                js = self._take_group(queue, used, clock() if realtime else 0.)
                 if js is None:
                     break
                 s = free.pop()
