@@ -152,17 +152,37 @@ __host__ __device__ inline void j012_asym_f32(float ax, float& j0, float& j1, fl
   j2 = -amp * (p2 * c - q2 * s);
 }
 
+// The float64 series branch of j012_f32, kept out of line: a kernel that
+// loops over many arguments would otherwise hold the 96 series
+// coefficients in registers across its loop.
+struct J012f {
+  float j0, j1, j2;
+};
+__host__ __device__ __noinline__ inline J012f j012_series_f32(float ax) {
+  double d0, d1, d2;
+  j012_abs((double)ax, d0, d1, d2);
+  return J012f{(float)d0, (float)d1, (float)d2};
+}
+
 // J_0, J_1, J_2 of a float32 argument as float32: the float64 power series
 // below 14 (where float32 would lose digits to cancellation), the float32
-// expansion above.
+// expansion above.  OUTLINE calls the series out of line (same values).
+template <bool OUTLINE = false>
 __host__ __device__ inline void j012_f32(float x, float& j0, float& j1, float& j2) {
   const float ax = fabsf(x);
   if (ax < 14.f) {
-    double d0, d1, d2;
-    j012_abs((double)ax, d0, d1, d2);
-    j0 = (float)d0;
-    j1 = (float)d1;
-    j2 = (float)d2;
+    if (OUTLINE) {
+      const J012f r = j012_series_f32(ax);
+      j0 = r.j0;
+      j1 = r.j1;
+      j2 = r.j2;
+    } else {
+      double d0, d1, d2;
+      j012_abs((double)ax, d0, d1, d2);
+      j0 = (float)d0;
+      j1 = (float)d1;
+      j2 = (float)d2;
+    }
   } else {
     j012_asym_f32(ax, j0, j1, j2);
   }

@@ -92,9 +92,11 @@ BPMH_DEV SampleParams sample_params(const DwArgs& a, int i_n, int i_t) {
 }
 
 // amplitudes w (cos t + 1) J0, w sin t J1, w (cos t - 1) J2 at kr sin t
+// (OUTLINE: the Bessel series out of line, same values)
+template <bool OUTLINE = false>
 BPMH_DEV Sample sample_amps(SampleParams p, float kr) {
   float j0, j1, j2;
-  bpmbessel::j012_f32(f_mul(kr, p.si), j0, j1, j2);
+  bpmbessel::j012_f32<OUTLINE>(f_mul(kr, p.si), j0, j1, j2);
   Sample s;
   s.co = p.co;
   s.a0 = f_mul(f_mul(p.w, f_add(p.co, 1.f)), j0);
@@ -179,24 +181,27 @@ BPMH_DEV void pair_ij(int p, int& i, int& j) {
 // both lateral points (i, j) and (j, i) of pair p; npairs = nx (nx + 1) / 2
 // columns, zero-padded to cp.
 
-// All tables of one grid call in a single launch, block row s = sample s:
-// blocks x < nbx fill the amplitudes A_n[s][col] (column pairs when PAIRS),
-// block nbx the phases P[s][z] = e^{i kz cos t_s}, and block nbx + 1 the
-// azimuth terms of the lateral points (when trig is set; points s*256 ..,
-// strided by the rows).  The sample's parameters (cos t, sin t, w) are
-// evaluated once per block; every value equals the one-kernel-per-table
-// form bit for bit.
-template <bool PAIRS>
+// All tables of one grid call in a single launch, block row r = samples
+// SB*r .. SB*r+SB-1: blocks x < nbx fill the amplitudes A_n[s][col] (column
+// pairs when PAIRS), block nbx the phases P[s][z] = e^{i kz cos t_s}, and
+// block nbx + 1 the azimuth terms of the lateral points (when trig is set;
+// points r*256 .., strided by the rows).  The SB samples' parameters (cos t,
+// sin t, w) are evaluated once per block by SB threads in parallel and each
+// thread's lateral point once for its SB samples: with one sample per block
+// the serial float64 parameter evaluation before the block's barrier bound
+// the launch.  Every value equals the one-kernel-per-table form bit for bit.
+template <bool PAIRS, int SB>
 __global__ void __launch_bounds__(256) k_dw_tables(DwArgs a, int total, int ncols, int tp, int cp,
                                                    int zp, float* __restrict__ A,
                                                    float2* __restrict__ P,
                                                    float4* __restrict__ trig) {
-  const int s = blockIdx.y;
+  const int s0 = (int)blockIdx.y * SB;
   const int nbx = (cp + 255) / 256;
   const int bx = blockIdx.x;
   if (bx == nbx + 1) {
     if (!trig) return;
-    for (int k = s * 256 + (int)threadIdx.x; k < a.nx * a.ny; k += tp * 256) {
+    for (int k = (int)blockIdx.y * 256 + (int)threadIdx.x; k < a.nx * a.ny;
+         k += (int)gridDim.y * 256) {
       const int i = k % a.nx, j = k / a.nx;
       const float phi = (float)atan2((double)axis_coord(j, a.ny, a.y1, a.y2),
                                      (double)axis_coord(i, a.nx, a.x1, a.x2));
@@ -205,48 +210,59 @@ __global__ void __launch_bounds__(256) k_dw_tables(DwArgs a, int total, int ncol
     }
     return;
   }
-  __shared__ float sp[3];
-  if (threadIdx.x == 0) {
+  __shared__ float sp[SB][3];
+  if (threadIdx.x < SB) {
+    const int s = s0 + (int)threadIdx.x;
     if (s < total) {
       const int per_ann = a.S + 1;
       const SampleParams p = sample_params(a, s / per_ann, s % per_ann);
-      sp[0] = p.co;
-      sp[1] = p.si;
-      sp[2] = p.w;
+      sp[threadIdx.x][0] = p.co;
+      sp[threadIdx.x][1] = p.si;
+      sp[threadIdx.x][2] = p.w;
     } else {
-      sp[0] = sp[1] = sp[2] = 0.f;
+      sp[threadIdx.x][0] = sp[threadIdx.x][1] = sp[threadIdx.x][2] = 0.f;
     }
   }
   __syncthreads();
   if (bx == nbx) {
-    for (int z = threadIdx.x; z < zp; z += blockDim.x) {
-      float2 v = make_float2(0.f, 0.f);
-      if (z < a.nz && s < total) {
-        const float kz = k_times(a.lam, axis_coord(z, a.nz, a.z1, a.z2));
-        sincosf(kz * sp[0], &v.y, &v.x);
+    for (int q = 0; q < SB; ++q) {
+      const int s = s0 + q;
+      for (int z = threadIdx.x; z < zp; z += blockDim.x) {
+        float2 v = make_float2(0.f, 0.f);
+        if (z < a.nz && s < total) {
+          const float kz = k_times(a.lam, axis_coord(z, a.nz, a.z1, a.z2));
+          sincosf(kz * sp[q][0], &v.y, &v.x);
+        }
+        P[(size_t)s * zp + z] = v;
       }
-      P[(size_t)s * zp + z] = v;
     }
     return;
   }
   const int col = bx * 256 + (int)threadIdx.x;
   if (col >= cp) return;
   const size_t plane = (size_t)tp * cp;
-  const size_t o = (size_t)s * cp + col;
-  if (s >= total || col >= ncols) {
-    A[o] = A[plane + o] = A[2 * plane + o] = 0.f;
-    return;
+  float kr = 0.f;
+  if (col < ncols) {
+    int i, j;
+    if (PAIRS) pair_ij(col, i, j);
+    else i = col % a.nx, j = col / a.nx;
+    const float x = axis_coord(i, a.nx, a.x1, a.x2);
+    const float y = axis_coord(j, a.ny, a.y1, a.y2);
+    kr = k_times(a.lam, __fsqrt_rn(f_add(f_mul(x, x), f_mul(y, y))));
   }
-  int i, j;
-  if (PAIRS) pair_ij(col, i, j);
-  else i = col % a.nx, j = col / a.nx;
-  const float x = axis_coord(i, a.nx, a.x1, a.x2);
-  const float y = axis_coord(j, a.ny, a.y1, a.y2);
-  const float kr = k_times(a.lam, __fsqrt_rn(f_add(f_mul(x, x), f_mul(y, y))));
-  const Sample sm = sample_amps(SampleParams{sp[0], sp[1], sp[2]}, kr);
-  A[o] = sm.a0;
-  A[plane + o] = sm.a1;
-  A[2 * plane + o] = sm.a2;
+#pragma unroll 1
+  for (int q = 0; q < SB; ++q) {
+    const int s = s0 + q;
+    const size_t o = (size_t)s * cp + col;
+    if (s >= total || col >= ncols) {
+      A[o] = A[plane + o] = A[2 * plane + o] = 0.f;
+      continue;
+    }
+    const Sample sm = sample_amps<true>(SampleParams{sp[q][0], sp[q][1], sp[q][2]}, kr);
+    A[o] = sm.a0;
+    A[plane + o] = sm.a1;
+    A[2 * plane + o] = sm.a2;
+  }
 }
 
 // MFMA form of the product (v_mfma_f32_16x16x4f32; wave64 operand layout:
@@ -528,26 +544,27 @@ __global__ void __launch_bounds__(128) k_debye_mirror2(int nx0, int ny0, int nz0
 }
 
 // The spread from the pair-indexed integral table (identical octant axes).
-// A work-group takes `rows` octant rows oy0.. (all ox) of octant plane oz
-// for one full-grid plane fz (oz = |fz - nz0/2|): it first gathers their
-// integrals into LDS -- pair (min, max) of point (ox, oy), rows i <= j read
-// along ox and the transposed points along oy, so both are runs of
-// consecutive pairs -- then writes the full-grid rows fy = ny0/2 +- oy, two
-// x-neighbours per lane with 16-byte streaming stores.  Each voxel gets the
-// fields of its octant point (ox, oy, oz) formed with that point's azimuth
-// terms -- the values the octant would hold -- conjugated for fz <= nz0/2.
-__global__ void __launch_bounds__(256) k_debye_spread_pairs(DwArgs a, int nx0, int ny0, int nz0,
+// A work-group takes `rows` octant rows oy0.. (all ox) of octant plane
+// oz = blockIdx.z: it first gathers their integrals into LDS -- pair
+// (min, max) of point (ox, oy), rows i <= j read along ox and the
+// transposed points along oy, so both are runs of consecutive pairs -- then
+// forms each point's fields once and writes them to the full-grid rows
+// fy = ny0/2 +- oy of both images of the plane, fz = nz0/2 + oz (for
+// oz < nz0/2) and fz = nz0/2 - oz (for oz >= 1), two x-neighbours per lane
+// with 16-byte streaming stores.  Each voxel gets the fields of its octant
+// point (ox, oy, oz) formed with that point's azimuth terms -- the values
+// the octant would hold -- conjugated for fz <= nz0/2.
+__global__ void __launch_bounds__(512) k_debye_spread_pairs(DwArgs a, int nx0, int ny0, int nz0,
                                                             int rows,
                                                             const float2* __restrict__ itab,
                                                             const float4* __restrict__ trig,
                                                             float* u, float2* ex, float2* ey,
-                                                            float2* ez, PlaneMap pm) {
+                                                            float2* ez) {
   extern __shared__ float2 S[];  // [rows][nxo][3]
   const int nxo = nx0 / 2 + 1;
   const int npairs = nxo * (nxo + 1) / 2;
   const int oy0 = blockIdx.x * rows;
-  const int fz = plane_of(pm);
-  const int oz = abs(fz - nz0 / 2);
+  const int oz = blockIdx.z;
   const float2* it = itab + (size_t)oz * npairs * 3;
   const int nr = min(rows, nxo - oy0);
   // points with ox <= oy: pair tri(oy) + ox, consecutive along ox
@@ -571,8 +588,11 @@ __global__ void __launch_bounds__(256) k_debye_spread_pairs(DwArgs a, int nx0, i
     d[2] = q[2];
   }
   __syncthreads();
-  const bool conj = fz <= nz0 / 2;
-  const int hx = nx0 / 2, hy = ny0 / 2;
+  const int hx = nx0 / 2, hy = ny0 / 2, hz = nz0 / 2;
+  // the two images: fz = hz + oz (conjugated only for oz = 0) and hz - oz
+  const bool has_hi = oz < hz, has_lo = oz >= 1;
+  const bool conj_hi = oz == 0;
+  const size_t plane_hi = (size_t)(hz + oz) * ny0, plane_lo = (size_t)(hz - oz) * ny0;
   const int lanes_per_row = nx0 / 2;  // two x-neighbours per lane
   // full-grid rows of this block: fy = hy + oy (oy < hy) and hy - oy (oy >= 1)
   const int nrows = 2 * nr;
@@ -590,17 +610,27 @@ __global__ void __launch_bounds__(256) k_debye_spread_pairs(DwArgs a, int nx0, i
       const DwTrig tr{tv.x, tv.y, tv.z, tv.w};
       const float2* v = S + ((size_t)r * nxo + ox) * 3;
       o[h] = dw_fields(a, tr, v[0], v[1], v[2]);
-      if (conj) {
-        o[h].ex.y = -o[h].ex.y;
-        o[h].ey.y = -o[h].ey.y;
-        o[h].ez.y = -o[h].ez.y;
-      }
     }
-    const size_t sf = ((size_t)fz * ny0 + fy) * nx0 + fx;
-    if (u) __builtin_nontemporal_store(dw_f2v{o[0].I, o[1].I}, (dw_f2v*)(u + sf));
-    if (ex) __builtin_nontemporal_store(dw_f4v{o[0].ex.x, o[0].ex.y, o[1].ex.x, o[1].ex.y}, (dw_f4v*)(ex + sf));
-    if (ey) __builtin_nontemporal_store(dw_f4v{o[0].ey.x, o[0].ey.y, o[1].ey.x, o[1].ey.y}, (dw_f4v*)(ey + sf));
-    if (ez) __builtin_nontemporal_store(dw_f4v{o[0].ez.x, o[0].ez.y, o[1].ez.x, o[1].ez.y}, (dw_f4v*)(ez + sf));
+    const dw_f2v vu{o[0].I, o[1].I};
+    const dw_f4v vx{o[0].ex.x, o[0].ex.y, o[1].ex.x, o[1].ex.y};
+    const dw_f4v vy{o[0].ey.x, o[0].ey.y, o[1].ey.x, o[1].ey.y};
+    const dw_f4v vz{o[0].ez.x, o[0].ez.y, o[1].ez.x, o[1].ez.y};
+    const dw_f4v cj{1.f, -1.f, 1.f, -1.f};
+    if (has_hi) {
+      const size_t sf = (plane_hi + fy) * nx0 + fx;
+      const dw_f4v m = conj_hi ? cj : dw_f4v{1.f, 1.f, 1.f, 1.f};
+      if (u) __builtin_nontemporal_store(vu, (dw_f2v*)(u + sf));
+      if (ex) __builtin_nontemporal_store(vx * m, (dw_f4v*)(ex + sf));
+      if (ey) __builtin_nontemporal_store(vy * m, (dw_f4v*)(ey + sf));
+      if (ez) __builtin_nontemporal_store(vz * m, (dw_f4v*)(ez + sf));
+    }
+    if (has_lo) {
+      const size_t sf = (plane_lo + fy) * nx0 + fx;
+      if (u) __builtin_nontemporal_store(vu, (dw_f2v*)(u + sf));
+      if (ex) __builtin_nontemporal_store(vx * cj, (dw_f4v*)(ex + sf));
+      if (ey) __builtin_nontemporal_store(vy * cj, (dw_f4v*)(ey + sf));
+      if (ez) __builtin_nontemporal_store(vz * cj, (dw_f4v*)(ez + sf));
+    }
   }
 }
 
@@ -805,26 +835,28 @@ static void launch_mirror(int nx0, int ny0, int nz0, const float* uo, const floa
   }
 }
 
-// The spread of every octant plane from the pair-indexed integral table
-// (16-byte aligned Ex/Ey/Ez, 8-byte aligned I: the caller checks).
+// The spread of every octant plane (nz0/2 + 1 of them, each to its two
+// full-grid images) from the pair-indexed integral table (16-byte aligned
+// Ex/Ey/Ez, 8-byte aligned I: the caller checks).
 static void launch_spread_pairs(const DwArgs& a, int nx0, int ny0, int nz0, const float2* itab,
                                 const float4* trig, float* u, float2* ex, float2* ey, float2* ez,
                                 hipStream_t st) {
-  const int hz = nz0 / 2;
-  PlaneMap pm;
-  pm.hi0 = hz;         // images hz + o, o < hz
-  pm.n_hi = hz;
-  pm.lo0 = hz - 1;     // images hz - o, o >= 1
-  const int planes = nz0;
+  const int planes = nz0 / 2 + 1;
   const int nxo = nx0 / 2 + 1;
-  // up to 8 octant rows per work-group within 48 KiB of LDS (the caller
-  // keeps nx0 <= 2048, so at least one row fits)
+  // two octant rows (four full-grid rows of each image) per 512-thread
+  // work-group, one when two do not fit in 48 KiB of LDS (the caller keeps
+  // nx0 <= 2048, so one row fits): at 256^3 this measured 84 us against 92
+  // to 99 us for 1, 3, 4 or 8 rows, 128- or 256-thread groups, and for
+  // groups that write their image rows in memory order
   int rows = (int)((48 * 1024) / ((size_t)nxo * 3 * sizeof(float2)));
-  rows = rows < 1 ? 1 : (rows > 8 ? 8 : rows);
+  rows = rows < 1 ? 1 : (rows > 2 ? 2 : rows);
   const size_t lds = (size_t)rows * nxo * 3 * sizeof(float2);
-  hipLaunchKernelGGL(k_debye_spread_pairs, dim3((nxo + rows - 1) / rows, 1, planes), dim3(256),
-                     lds, st, a, nx0, ny0, nz0, rows, itab, trig, u, ex, ey, ez, pm);
+  hipLaunchKernelGGL(k_debye_spread_pairs, dim3((nxo + rows - 1) / rows, 1, planes), dim3(512),
+                     lds, st, a, nx0, ny0, nz0, rows, itab, trig, u, ex, ey, ez);
 }
+
+// samples per block row of the tables launch (divides 16)
+constexpr int DW_SB = 8;
 
 // Tables and product for the grid in `a` (octant or whole grid) on stream
 // st; with `full`, the octant is then spread to the full grid.
@@ -889,13 +921,13 @@ static int debye_grid_run(DwArgs& a, int int_steps, int n_alphas, hipStream_t st
     a.I = (float*)(oex + 3 * noct);
   }
   int rc = dw_timed(0, st, [&] {
-    const dim3 tgrid((cp + 255) / 256 + 2, tp);
+    const dim3 tgrid((cp + 255) / 256 + 2, tp / DW_SB);  // tp: a multiple of 16
     if (pairs)
-      hipLaunchKernelGGL(k_dw_tables<true>, tgrid, dim3(256), 0, st, a, total, ntab, tp, cp, zp, A,
-                         P, trig);
+      hipLaunchKernelGGL((k_dw_tables<true, DW_SB>), tgrid, dim3(256), 0, st, a, total, ntab, tp, cp,
+                         zp, A, P, trig);
     else
-      hipLaunchKernelGGL(k_dw_tables<false>, tgrid, dim3(256), 0, st, a, total, ncols, tp, cp, zp, A,
-                         P, trig);
+      hipLaunchKernelGGL((k_dw_tables<false, DW_SB>), tgrid, dim3(256), 0, st, a, total, ncols, tp,
+                         cp, zp, A, P, trig);
   });
   const dim3 mgrid((unsigned)mblocks);
   if (!rc) rc = dw_timed(1, st, [&] {

@@ -205,20 +205,22 @@ def test_plane_variant_fills_a_provided_tensor(bpm):
         Q.focus_field_debye_plane((32, 24), (.1, .1), ex_g=torch.zeros((24, 32), device="cuda"))
 
 
-@pytest.mark.parametrize("shape", [(64, 64, 64), (48, 32, 40), (256, 256, 256)])
+@pytest.mark.parametrize("shape", [(64, 64, 64), (48, 32, 40), (256, 256, 256), (128, 128, 200),
+                                   (96, 96, 512)])
 def test_fused_focus_field_equals_octant_plus_mirror(bpm, shape):
     """bpmh_focus_field_debye (the product's epilogue writes every octant
     value to its mirror positions) against bpmh_debye_wolf on the octant
     followed by bpmh_debye_mirror: the same values bit for bit, for the
     pair-table path (square x/y axes) and the general one, and at the 256^3
-    workload size."""
+    workload size, and square grids whose octant depth (101, 257 planes) is
+    not a whole number of the product's plane groups."""
     import torch
     from bpm_amd import focus_fields as Q
     from bpm_amd import _lib
     say("case fused Debye", shape)
     Nx0, Ny0, Nz0 = shape
     units, lam, NA, n0, S = (.05, .06, .05), .5, 1.1, 1.33, 128
-    if shape == (64, 64, 64) or shape == (256, 256, 256):
+    if Nx0 == Ny0:
         units = (.05,) * 3
     dx, dy, dz = units
     alphas = Q._alphas(NA, n0)
