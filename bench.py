@@ -112,25 +112,35 @@ def cpu_baseline(u0, dn, units, lam, n0, planes, planes_1core):
 
 def debye_extra(torch):
     """The 256^3 Debye-Wolf workload (NA 1.1, n0 1.33, 512 theta steps): the
-    whole call (octant quadrature written straight to the full grid), then
-    one call with HIP events around each of its kernels for the product's
-    MFMA rate."""
+    whole call (octant quadrature written straight to the full grid) --
+    calls issued back to back (wall time and one event pair over all of
+    them, per call) and single calls each waited for (host included) --
+    then one call with HIP events around each of its kernels for the
+    product's MFMA rate."""
     import ctypes
     from bpm_amd import focus_fields as Q
     from bpm_amd import _lib
     shape, units, lam, NA, n0, S = (256, 256, 256), (.05,) * 3, .5, 1.1, 1.33, 512
-    Q.focus_field_debye_device(shape, units, lam, NA, n0, S)
-    torch.cuda.synchronize()
-    reps = 5
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * reps)]
-    t0 = time.perf_counter()
-    for r in range(reps):
-        ev[2 * r].record()
+    for _ in range(2):
         Q.focus_field_debye_device(shape, units, lam, NA, n0, S)
-        ev[2 * r + 1].record()
+    torch.cuda.synchronize()
+    reps = 20
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for r in range(reps):
+        Q.focus_field_debye_device(shape, units, lam, NA, n0, S)
+    e1.record()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / reps
-    gpu_ms = sum(ev[2 * r].elapsed_time(ev[2 * r + 1]) for r in range(reps)) / reps
+    gpu_ms = e0.elapsed_time(e1) / reps
+    single = []
+    for r in range(5):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        Q.focus_field_debye_device(shape, units, lam, NA, n0, S)
+        torch.cuda.synchronize()
+        single.append(time.perf_counter() - t1)
     lib = _lib.load()
     _lib.check(lib.bpmh_debye_set_timing(1))
     Q.focus_field_debye_device(shape, units, lam, NA, n0, S)
@@ -149,11 +159,16 @@ def debye_extra(torch):
     useful_tf = 12. * nz * npairs * total / prod_s / 1e12
     samples = 129 ** 3 * (S + 1)
     return {"workload": "Debye-Wolf 256^3, NA 1.1, n0 1.33, 512 theta steps, x-polarised",
-            "ms": dt * 1e3, "ms_gpu_span": gpu_ms, "theta_samples_per_s": samples / dt,
+            "ms": dt * 1e3, "ms_gpu_span": gpu_ms, "ms_single_call": float(np.median(single)) * 1e3,
+            "theta_samples_per_s": samples / dt,
             "kernel_us": {"tables": ms[0] * 1e3, "product": ms[1] * 1e3,
                           "full_grid_spread": ms[2] * 1e3},
             "product_mfma_tflops": mfma_tf, "product_mfma_frac": mfma_tf / FP32_PEAK_TFLOPS,
             "product_useful_tflops": useful_tf,
+            "timing": "ms = wall time of %d calls issued back to back / %d (GPU-bound: the host "
+                      "part of a call is ~30 us); ms_gpu_span = one event pair around them / %d; "
+                      "ms_single_call = median of 5 calls each synchronised before and after "
+                      "(host launch and completion wait included)" % (reps, reps, reps),
             "note": "MFMA fraction = flops the MFMA units execute over the padded "
                     "144 x 8416 x 528 product (12 per plane, column pair and sample) / the "
                     "product kernel's time (event pair) / 157.3 TFLOP/s dense FP32 matrix "
