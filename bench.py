@@ -151,9 +151,9 @@ def debye_extra(torch):
     # the product: for every (plane, table column, sample) 6 real
     # multiply-adds (Re and Im of I0, I1, I2), 12 flop; the table runs over
     # the 129*130/2 column pairs of the 129^2 octant plane and is padded to
-    # whole MFMA tiles (planes to 144, columns to 8416, samples to 528)
+    # whole MFMA tiles (planes to 144, columns to 8400, samples to 528)
     nz, npairs, total = 129, 129 * 130 // 2, S + 1
-    zp, cp, tp = 144, (npairs + 31) // 32 * 32, (total + 15) // 16 * 16
+    zp, cp, tp = 144, (npairs + 15) // 16 * 16, (total + 15) // 16 * 16
     prod_s = ms[1] * 1e-3
     mfma_tf = 12. * zp * cp * tp / prod_s / 1e12
     useful_tf = 12. * nz * npairs * total / prod_s / 1e12
@@ -170,7 +170,7 @@ def debye_extra(torch):
                       "ms_single_call = median of 5 calls each synchronised before and after "
                       "(host launch and completion wait included)" % (reps, reps, reps),
             "note": "MFMA fraction = flops the MFMA units execute over the padded "
-                    "144 x 8416 x 528 product (12 per plane, column pair and sample) / the "
+                    "144 x 8400 x 528 product (12 per plane, column pair and sample) / the "
                     "product kernel's time (event pair) / 157.3 TFLOP/s dense FP32 matrix "
                     "peak; useful = unpadded 129 x 8385 x 513"}
 

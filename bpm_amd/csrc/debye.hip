@@ -274,13 +274,15 @@ __global__ void __launch_bounds__(256) k_dw_tables(DwArgs a, int total, int ncol
 // accumulators per tile pair: Re, Im of I0, I1, I2) and reads its operands
 // straight from the tables into registers, the next 16 samples in flight
 // while the current ones are multiplied; no LDS and no barriers, so
-// wavefronts run independently.  The four wavefronts of a work-group share
-// their table columns (their right operands hit in L1).  (Measured
-// alternatives at 256^3 with the pair table: two plane tiles per wavefront
-// 6 % slower, one column tile per wavefront 5 % slower; earlier,
-// sample-contiguous tables with one 16-byte operand load per row and batch
-// were 6 % slower.)
-constexpr int DM_CT = 2;  // 16-column tiles per wavefront
+// wavefronts run independently.  The wavefronts of a work-group share
+// their table columns (their right operands hit in L1).  One column tile
+// per wavefront: 74 registers, so the 256^3 octant's 4752 wavefronts are
+// all resident at 4-5 per SIMD (89 us); two column tiles need 126
+// registers and give 2376 wavefronts at 2-3 per SIMD, the busiest SIMDs
+// then holding 1.3x the mean work (107 us).  (Also measured at 256^3: two
+// plane tiles per wavefront 6 % slower; sample-contiguous tables with one
+// 16-byte operand load per row and batch 6 % slower.)
+constexpr int DM_CT = 1;  // 16-column tiles per wavefront
 typedef float dm_v4f __attribute__((ext_vector_type(4)));
 typedef unsigned dw_u2v __attribute__((ext_vector_type(2)));
 
@@ -429,15 +431,15 @@ static int dw_waves_per_group(int nzt) {
   return best;
 }
 
-// Wavefront w of a work-group of W wavefronts (W = blockDim.x / 64) at (x, y)
-// owns the 16-plane tile W*y + w and the table columns 32x .. 32x+31.  W is
-// chosen by the host so that W divides the number of plane tiles when it
-// can: every wavefront then has work and, with three wavefronts per SIMD
-// (154 registers), all work-groups of a 256^3 call are resident at once
-// (four-wavefront groups over nine plane tiles left 21 groups for a second
-// round).  (Two plane tiles per wavefront reuse each loaded column operand
-// twice but need 220 registers, two wavefronts per SIMD, and leave a tail of
-// unequal wavefronts: 0.364 vs 0.341 ms per 256^3 call.)
+// Wavefront w of a work-group of W wavefronts (W = blockDim.x / 64) owns
+// the 16-plane tile W*g + w of its plane group g and the 16*DM_CT table
+// columns of its column block.  W is chosen by the host so that W divides
+// the number of plane tiles when it can: every wavefront then has work and
+// all work-groups of a 256^3 call are resident at once (four-wavefront
+// groups over nine plane tiles had left groups for a second round).  (Two
+// plane tiles per wavefront reuse each loaded column operand twice but
+// need more registers, fewer wavefronts per SIMD, and leave a tail of
+// unequal wavefronts: 0.364 vs 0.341 ms per 256^3 call when measured.)
 template <bool PAIRS, bool ITAB>
 __global__ void __launch_bounds__(256, 2) k_dw_mfma(DwArgs a, int ncols, int nzt, int tp, int cp,
                                                 const float2* __restrict__ P,
