@@ -46,16 +46,18 @@ def _grid_call(Nx, Ny, Nz, x1, x2, y1, y2, z1, z2, lam, alphas, steps, variant, 
 
 
 def focus_field_debye_device(shape, units, lam, NA, n0=1., n_integration_steps=200):
-    """focus_field_debye with the result left on the GPU as torch tensors (u, ex, ey, ez)."""
+    """focus_field_debye with the result left on the GPU as torch tensors
+    (u, ex, ey, ez).  The four are views of one allocation (Ex, Ey, Ez, then
+    I; each 16-byte aligned), which saves three allocator calls per call."""
     Nx0, Ny0, Nz0 = shape
     dx, dy, dz = units
     if not Nx0 % 2 + Ny0 % 2 + Nz0 % 2 == 0:
         raise NotImplementedError("odd shapes not supported yet")
     alphas = _alphas(NA, n0)
-    u = _lib.empty((Nz0, Ny0, Nx0), np.float32)
-    ex = _lib.empty((Nz0, Ny0, Nx0), np.complex64)
-    ey = _lib.empty((Nz0, Ny0, Nx0), np.complex64)
-    ez = _lib.empty((Nz0, Ny0, Nx0), np.complex64)
+    n = Nz0 * Ny0 * Nx0  # even: every axis is
+    buf = _lib.empty((3 * n + n // 2,), np.complex64)
+    ex, ey, ez = (buf[i * n:(i + 1) * n].view(Nz0, Ny0, Nx0) for i in range(3))
+    u = buf[3 * n:].view(_lib.torch().float32).view(Nz0, Ny0, Nx0)
     a, ap = _alpha_arg(alphas)
     # the octant's quadrature with each value written to its mirror positions
     _lib.call("bpmh_focus_field_debye", int(Nx0), int(Ny0), int(Nz0), float(dx), float(dy),
