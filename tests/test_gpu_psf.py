@@ -236,3 +236,49 @@ def test_fused_focus_field_equals_octant_plus_mirror(bpm, shape):
     torch.cuda.synchronize()
     for g, r in zip(got, ref):
         assert torch.equal(g, r)
+
+
+def test_back_to_back_calls_equal_synchronised_calls(bpm):
+    """The grid path's calls share one library scratch buffer, ordered by an
+    event (calls on another stream wait for its last use; larger calls grow
+    it).  Calls issued back to back (fused full-grid calls of different
+    sizes, NA and steps, an octant-only call, a call on another stream) must
+    give the same bits as the same calls each synchronised."""
+    import torch
+    from bpm_amd import focus_fields as Q
+    from bpm_amd import _lib
+    say("case Debye back to back")
+    cfgs = [((64, 64, 64), .05, 1.1, 128), ((96, 96, 48), .04, .8, 64),
+            ((128, 128, 32), .06, 1.2, 96), ((64, 64, 64), .05, .5, 200)]
+
+    def full(c):
+        shape, d, NA, S = c
+        return Q.focus_field_debye_device(shape, (d,) * 3, .5, NA, 1.33, S)
+
+    def octant():
+        return Q._grid_call(33, 33, 17, 0., 1.6, 0., 1.6, 0., .8, .5 / 1.33, Q._alphas(.9, 1.33),
+                            80, _lib.DW_INDEX)
+
+    ref = []
+    for c in cfgs:
+        ref.append([t.clone() for t in full(c)])
+        torch.cuda.synchronize()
+    ref_oct = [t.clone() for t in octant()]
+    torch.cuda.synchronize()
+    got = []
+    for rep in range(2):
+        for c in cfgs:
+            got.append(full(c))
+        got.append(octant())
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        other = full(cfgs[2])
+    s.synchronize()
+    torch.cuda.synchronize()
+    want = (ref + [ref_oct]) * 2
+    for g, r in zip(got, want):
+        for a, b in zip(g, r):
+            assert torch.equal(a, b)
+    for a, b in zip(other, ref[2]):
+        assert torch.equal(a, b)
+
