@@ -652,12 +652,45 @@ k_col(ColArgs a) {
         an = n_term_f32(n, a.k0);
         kx2 = kcomp2_f32(x, a.nx, a.dx);
       }
+      if constexpr (PHASE == 2) {
+        // weighted pass B: H(n0 + dn0) on the fly for every row.  H(ky) ==
+        // H(NY - ky) bit for bit (kcomp2_f32 of both rows is the same
+        // float), and the rows of thread t mirror those of thread T - t
+        // (element m <-> E-1-m; thread 0 mirrors its own, m <-> E-m, and
+        // holds row NY/2 alone), so each thread evaluates its first E/2
+        // rows (thread 0 also row NY/2) and takes the others from its
+        // partner through the column's LDS line (free before the inverse
+        // transform).
+        float2 h[E];
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const int ky = t + m * T;
-        const float2 h =
-            (a.hmode == 0) ? hv[q][m] : prop_f32(kx2, kcomp2_f32(ky, NY, a.dy), an, a.dz);
-        v[q][m] = a.conj_h ? c_mulc(v[q][m], h) : c_mul(v[q][m], h);
+        for (int m = 0; m < E / 2; ++m)
+          h[m] = prop_f32(kx2, kcomp2_f32(t + m * T, NY, a.dy), an, a.dz);
+        if (t == 0) h[E / 2] = prop_f32(kx2, kcomp2_f32(NY / 2, NY, a.dy), an, a.dz);
+        __syncthreads();  // earlier users of the line are done with it
+#pragma unroll
+        for (int m = 0; m < E / 2; ++m) line[m * T + t] = h[m];
+        __syncthreads();
+        const int pt = T - t;
+#pragma unroll
+        for (int m = E / 2; m < E; ++m) {
+          if (t == 0) {
+            if (m > E / 2) h[m] = h[E - m];
+          } else {
+            h[m] = line[(E - 1 - m) * T + pt];
+          }
+        }
+        __syncthreads();  // before the line is written again
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+          v[q][m] = a.conj_h ? c_mulc(v[q][m], h[m]) : c_mul(v[q][m], h[m]);
+      } else {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const int ky = t + m * T;
+          const float2 h =
+              (a.hmode == 0) ? hv[q][m] : prop_f32(kx2, kcomp2_f32(ky, NY, a.dy), an, a.dz);
+          v[q][m] = a.conj_h ? c_mulc(v[q][m], h) : c_mul(v[q][m], h);
+        }
       }
       if (PHASE == 4 && !a.simple) {
         // dn0 = sum|U| d / sum|U| depends on U through |U|:
