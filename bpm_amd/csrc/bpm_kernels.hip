@@ -471,13 +471,15 @@ struct ColArgs {
 // PHASE 4: dL/ddn0 from the partials, conj(H) X plus the term of dn0's
 //          dependence on |U|, the weight-slice gradient, IFFT_y (simple mean:
 //          conj(H) X, IFFT_y, dL/ddn0 stored).
+// PHASE 5: PHASE 0 with H evaluated on the fly (hmode 1, the simple mean),
+//          half the rows per thread as in PHASE 2.
 template <int NY, int V, int PHASE, int EE>
 __global__ void __launch_bounds__((ColCfg<NY, V, EE>::THREADS))
 k_col(ColArgs a) {
   using Cfg = ColCfg<NY, V, EE>;
   constexpr int E = Cfg::E, T = Cfg::T, COLS = Cfg::COLS;
-  constexpr bool FWD = PHASE == 0 || PHASE == 1 || PHASE == 3;  // forward FFT_y
-  constexpr bool INV = PHASE == 0 || PHASE == 2 || PHASE == 4;  // H, inverse FFT_y
+  constexpr bool FWD = PHASE == 0 || PHASE == 1 || PHASE == 3 || PHASE == 5;  // forward FFT_y
+  constexpr bool INV = PHASE == 0 || PHASE == 2 || PHASE == 4 || PHASE == 5;  // H, inverse FFT_y
   constexpr bool ADJW = PHASE == 3 || PHASE == 4;
   // each thread holds P adjacent columns; the LC lane columns of a 16-lane
   // group are the LDS lines it touches together
@@ -600,7 +602,7 @@ k_col(ColArgs a) {
 #pragma unroll
     for (int q = 0; q < P; ++q) line_fft<NY, E, false, CL>(v[q], lds[q * LC + c], t, tw);
   }
-  if constexpr (PHASE == 0) {
+  if constexpr (PHASE == 0 || PHASE == 5) {
     // forward recorded for the adjoint: U = F u of this step, before H
     if (a.kout) {
       float2* ko = a.kout + (size_t)b * a.kout_bstride + x0;
@@ -652,10 +654,11 @@ k_col(ColArgs a) {
         an = n_term_f32(n, a.k0);
         kx2 = kcomp2_f32(x, a.nx, a.dx);
       }
-      if constexpr (PHASE == 2) {
-        // weighted pass B: H(n0 + dn0) on the fly for every row.  H(ky) ==
-        // H(NY - ky) bit for bit (kcomp2_f32 of both rows is the same
-        // float), and the rows of thread t mirror those of thread T - t
+      if constexpr (PHASE == 2 || PHASE == 5) {
+        // H(n0 + dn0) (weighted pass B) or H(n_eff[z]) (simple mean) on the
+        // fly for every row.  H(ky) == H(NY - ky) bit for bit (kcomp2_f32
+        // of both rows is the same float), and the rows of thread t mirror
+        // those of thread T - t
         // (element m <-> E-1-m; thread 0 mirrors its own, m <-> E-m, and
         // holds row NY/2 alone), so each thread evaluates its first E/2
         // rows (thread 0 also row NY/2) and takes the others from its
@@ -1385,7 +1388,9 @@ static int prop_run(bpmh_plan plan, const PropCall& c, const StepScratch& sc, hi
   const int batch = c.batch, nz = c.nz;
   const row_fn row = pick_row(nx, plan->row_v);
   const int cv = col_variant_for(plan->col_v, c.plane);
-  const col_fn col0 = pick_col<0>(ny, cv);
+  // the simple mean's propagator is evaluated in the column pass (PHASE 5)
+  const col_fn col0 =
+      c.mean_mode == BPMH_MEAN_SIMPLE ? pick_col<5>(ny, cv) : pick_col<0>(ny, cv);
   const col_fn colA = pick_col<1>(ny, cv);
   const col_fn colB = pick_col<2>(ny, cv);
 
@@ -2164,7 +2169,7 @@ extern "C" int bpmh_mean_segment(bpmh_plan plan, int mean_mode, int batch, int z
   float2* plane = (float2*)plane_;
   float2* fp = adjoint ? (float2*)(work + L.fplane) : plane;
   const int cv = col_variant_for(plan->col_v, fp);
-  const col_fn col0 = pick_col<0>(ny, cv);
+  const col_fn col0 = simple ? pick_col<5>(ny, cv) : pick_col<0>(ny, cv);
   const col_fn colA = pick_col<1>(ny, cv);
   const col_fn colB = pick_col<2>(ny, cv);
   const size_t B = (size_t)batch, S = (size_t)nseg;
