@@ -634,15 +634,40 @@ k_col(ColArgs a) {
       // dH/ddn0 = H * (-i dz (n0+dn0) k0^2 / beta): collect Im(conj(X) H U)/beta
       const float an = n_term_f32(n_adj, a.k0);
       const float kx2 = kcomp2_f32(x, a.nx, a.dx);
-      float acc = 0.f;
+      // H and beta for half the rows, the mirrored rows (same values bit for
+      // bit) from the partner thread through the line, as in the H passes
+      float2 h[E];
+      float bt[E];
+#pragma unroll
+      for (int m = 0; m < E / 2; ++m)
+        h[m] = prop_f32_beta(kx2, kcomp2_f32(t + m * T, NY, a.dy), an, a.dz, &bt[m]);
+      if (t == 0)
+        h[E / 2] = prop_f32_beta(kx2, kcomp2_f32(NY / 2, NY, a.dy), an, a.dz, &bt[E / 2]);
+      __syncthreads();  // earlier users of the line are done with it
+#pragma unroll
+      for (int m = 0; m < E / 2; ++m) {
+        line[m * T + t] = h[m];
+        line[NY / 2 + m * T + t] = make_float2(bt[m], 0.f);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = E / 2; m < E; ++m) {
+        if (t == 0) {
+          if (m > E / 2) {
+            h[m] = h[E - m];
+            bt[m] = bt[E - m];
+          }
+        } else {
+          h[m] = line[(E - 1 - m) * T + T - t];
+          bt[m] = line[NY / 2 + (E - 1 - m) * T + T - t].x;
+        }
+      }
+      float acc = 0.f;  // (column_sum2 waits for every reader of the line)
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        const int ky = t + m * T;
-        float beta;
-        const float2 h = prop_f32_beta(kx2, kcomp2_f32(ky, NY, a.dy), an, a.dz, &beta);
-        if (beta > 0.f) {
-          const float2 hu = c_mul(h, uv[q][m]);
-          acc += (v[q][m].x * hu.y - v[q][m].y * hu.x) / beta;
+        if (bt[m] > 0.f) {
+          const float2 hu = c_mul(h[m], uv[q][m]);
+          acc += (v[q][m].x * hu.y - v[q][m].y * hu.x) / bt[m];
         }
       }
       const float2 tot = column_sum2<T>(make_float2(acc, 0.f), line, t);
@@ -654,9 +679,9 @@ k_col(ColArgs a) {
         an = n_term_f32(n, a.k0);
         kx2 = kcomp2_f32(x, a.nx, a.dx);
       }
-      if constexpr (PHASE == 2 || PHASE == 5) {
-        // H(n0 + dn0) (weighted pass B) or H(n_eff[z]) (simple mean) on the
-        // fly for every row.  H(ky) == H(NY - ky) bit for bit (kcomp2_f32
+      if constexpr (PHASE == 2 || PHASE == 4 || PHASE == 5) {
+        // H(n0 + dn0) (weighted pass B, its adjoint) or H(n_eff[z]) (simple
+        // mean, its adjoint) on the fly for every row.  H(ky) == H(NY - ky) bit for bit (kcomp2_f32
         // of both rows is the same float), and the rows of thread t mirror
         // those of thread T - t
         // (element m <-> E-1-m; thread 0 mirrors its own, m <-> E-m, and
