@@ -92,11 +92,9 @@ BPMH_DEV SampleParams sample_params(const DwArgs& a, int i_n, int i_t) {
 }
 
 // amplitudes w (cos t + 1) J0, w sin t J1, w (cos t - 1) J2 at kr sin t
-// (OUTLINE: the Bessel series out of line, same values)
-template <bool OUTLINE = false>
 BPMH_DEV Sample sample_amps(SampleParams p, float kr) {
   float j0, j1, j2;
-  bpmbessel::j012_f32<OUTLINE>(f_mul(kr, p.si), j0, j1, j2);
+  bpmbessel::j012_f32(f_mul(kr, p.si), j0, j1, j2);
   Sample s;
   s.co = p.co;
   s.a0 = f_mul(f_mul(p.w, f_add(p.co, 1.f)), j0);
@@ -210,17 +208,22 @@ __global__ void __launch_bounds__(256) k_dw_tables(DwArgs a, int total, int ncol
     }
     return;
   }
-  __shared__ float sp[SB][3];
+  // per sample: cos t, sin t, and the amplitude factors w (cos t + 1),
+  // w sin t, w (cos t - 1) of sample_amps (the same rounded products)
+  __shared__ float sp[SB][5];
   if (threadIdx.x < SB) {
     const int s = s0 + (int)threadIdx.x;
+    float* q = sp[threadIdx.x];
     if (s < total) {
       const int per_ann = a.S + 1;
       const SampleParams p = sample_params(a, s / per_ann, s % per_ann);
-      sp[threadIdx.x][0] = p.co;
-      sp[threadIdx.x][1] = p.si;
-      sp[threadIdx.x][2] = p.w;
+      q[0] = p.co;
+      q[1] = p.si;
+      q[2] = f_mul(p.w, f_add(p.co, 1.f));
+      q[3] = f_mul(p.w, p.si);
+      q[4] = f_mul(p.w, f_sub(p.co, 1.f));
     } else {
-      sp[threadIdx.x][0] = sp[threadIdx.x][1] = sp[threadIdx.x][2] = 0.f;
+      q[0] = q[1] = q[2] = q[3] = q[4] = 0.f;
     }
   }
   __syncthreads();
@@ -258,10 +261,11 @@ __global__ void __launch_bounds__(256) k_dw_tables(DwArgs a, int total, int ncol
       A[o] = A[plane + o] = A[2 * plane + o] = 0.f;
       continue;
     }
-    const Sample sm = sample_amps<true>(SampleParams{sp[q][0], sp[q][1], sp[q][2]}, kr);
-    A[o] = sm.a0;
-    A[plane + o] = sm.a1;
-    A[2 * plane + o] = sm.a2;
+    float j0, j1, j2;
+    bpmbessel::j012_f32<true>(f_mul(kr, sp[q][1]), j0, j1, j2);
+    A[o] = f_mul(sp[q][2], j0);
+    A[plane + o] = f_mul(sp[q][3], j1);
+    A[2 * plane + o] = f_mul(sp[q][4], j2);
   }
 }
 
