@@ -206,7 +206,7 @@ def test_plane_variant_fills_a_provided_tensor(bpm):
 
 
 @pytest.mark.parametrize("shape", [(64, 64, 64), (48, 32, 40), (256, 256, 256), (128, 128, 200),
-                                   (96, 96, 512), (2048, 2048, 4)])
+                                   (96, 96, 512), (2048, 2048, 4), (2, 2, 2), (4, 4, 6)])
 def test_fused_focus_field_equals_octant_plus_mirror(bpm, shape):
     """bpmh_focus_field_debye (the product's epilogue writes every octant
     value to its mirror positions) against bpmh_debye_wolf on the octant
@@ -214,7 +214,8 @@ def test_fused_focus_field_equals_octant_plus_mirror(bpm, shape):
     pair-table path (square x/y axes) and the general one, and at the 256^3
     workload size, and square grids whose octant depth (101, 257 planes) is
     not a whole number of the product's plane groups; 2048^2 octant rows
-    (1025 points) take the spread's one-row work-groups."""
+    (1025 points) take the spread's one-row work-groups; 2^3 and 4^2 x 6 are
+    the smallest grids (one lane per full-grid row pair)."""
     import torch
     from bpm_amd import focus_fields as Q
     from bpm_amd import _lib
