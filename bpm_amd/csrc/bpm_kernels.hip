@@ -681,14 +681,13 @@ k_col(ColArgs a) {
       }
       if constexpr (PHASE == 2 || PHASE == 4 || PHASE == 5) {
         // H(n0 + dn0) (weighted pass B, its adjoint) or H(n_eff[z]) (simple
-        // mean, its adjoint) on the fly for every row.  H(ky) == H(NY - ky) bit for bit (kcomp2_f32
-        // of both rows is the same float), and the rows of thread t mirror
-        // those of thread T - t
-        // (element m <-> E-1-m; thread 0 mirrors its own, m <-> E-m, and
-        // holds row NY/2 alone), so each thread evaluates its first E/2
-        // rows (thread 0 also row NY/2) and takes the others from its
-        // partner through the column's LDS line (free before the inverse
-        // transform).
+        // mean, its adjoint) on the fly for every row.  H(ky) == H(NY - ky)
+        // bit for bit (kcomp2_f32 of both rows is the same float), and the
+        // rows of thread t mirror those of thread T - t (element m <->
+        // E-1-m; thread 0 mirrors its own, m <-> E-m, and holds row NY/2
+        // alone), so each thread evaluates its first E/2 rows (thread 0 also
+        // row NY/2) and takes the others from its partner through the
+        // column's LDS line (free before the inverse transform).
         float2 h[E];
 #pragma unroll
         for (int m = 0; m < E / 2; ++m)
