@@ -184,10 +184,12 @@ BPMH_DEV void pair_ij(int p, int& i, int& j) {
 // pairs when PAIRS), block nbx the phases P[s][z] = e^{i kz cos t_s}, and
 // block nbx + 1 the azimuth terms of the lateral points (when trig is set;
 // points r*256 .., strided by the rows).  The SB samples' parameters (cos t,
-// sin t, w) are evaluated once per block by SB threads in parallel and each
-// thread's lateral point once for its SB samples: with one sample per block
-// the serial float64 parameter evaluation before the block's barrier bound
-// the launch.  Every value equals the one-kernel-per-table form bit for bit.
+// sin t, w) are evaluated once per block by SB threads in parallel, each
+// thread's lateral point (kr) once for its SB samples, and the float64
+// Bessel series is called out of line (inlined in the sample loop, its 96
+// coefficients were kept in registers: 212 instead of 88).  The launch is
+// bound by its vector instruction issue.  Every value equals the
+// one-kernel-per-table form bit for bit.
 template <bool PAIRS, int SB>
 __global__ void __launch_bounds__(256) k_dw_tables(DwArgs a, int total, int ncols, int tp, int cp,
                                                    int zp, float* __restrict__ A,
