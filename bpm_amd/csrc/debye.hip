@@ -864,7 +864,7 @@ static void launch_spread_pairs(const DwArgs& a, int nx0, int ny0, int nz0, cons
 }
 
 // samples per block row of the tables launch (divides 16)
-constexpr int DW_SB = 8;
+constexpr int DW_SB = 4;
 
 // Tables and product for the grid in `a` (octant or whole grid) on stream
 // st; with `full`, the octant is then spread to the full grid.
