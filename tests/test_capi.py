@@ -67,6 +67,19 @@ def test_argument_validation_paths(lib):
         assert b"too large" in lib.bpmh_last_error()
     rc = lib.bpmh_debye_mirror(9, 8, 8, None, None, None, None, None, None, None, None, None)
     assert rc == _lib.ERR_UNSUPPORTED
+    # the one-call full-grid entry: odd shapes (as _focus_fields_debye.py:81-82),
+    # a grid beyond the launch limits and missing aperture angles are refused
+    # before any device call
+    al = np.array([0., .5], np.float32)
+    rc = lib.bpmh_focus_field_debye(9, 8, 8, .1, .1, .1, .5, al.ctypes.data, 2, 64,
+                                    None, None, None, None, None)
+    assert rc == _lib.ERR_UNSUPPORTED and b"odd" in lib.bpmh_last_error()
+    rc = lib.bpmh_focus_field_debye(8, 8, 1 << 17, .1, .1, .1, .5, al.ctypes.data, 2, 64,
+                                    None, None, None, None, None)
+    assert rc == _lib.ERR_INVALID and b"too large" in lib.bpmh_last_error()
+    rc = lib.bpmh_focus_field_debye(8, 8, 8, .1, .1, .1, .5, None, 2, 64,
+                                    None, None, None, None, None)
+    assert rc == _lib.ERR_INVALID
     rc = lib.bpmh_bessel_fill(3, None, None, 4, None)
     assert rc == _lib.ERR_UNSUPPORTED
     with pytest.raises(ValueError):
